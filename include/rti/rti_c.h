@@ -1,0 +1,176 @@
+/*
+ * rti_c.h -- C ABI of the MI355X (gfx950) Monte Carlo flux tracer.
+ *
+ * This is the boundary the C++ drop-in `rti::device` (include/rti/device.hpp)
+ * and any other host binding (ctypes, cgo, JNI) call. Plain pointers and
+ * sizes only; the library copies inputs in and results out, owns all device
+ * memory, and every call is synchronous on return.
+ *
+ * Which reference interface each entry point replaces (reference paths are
+ * relative to the rti source tree):
+ *   rti_ctx_create / rti_ctx_destroy  -> rti::device::device() and the per-run
+ *                                        rtcNewDevice/rtcReleaseDevice
+ *                                        (src/rti/device.hpp:42-45, 84-85, 101)
+ *   rti_set_discs                     -> device::set_points + set_normals +
+ *                                        set_grid_spacing, packed as
+ *                                        (x,y,z,r) by
+ *                                        combine_points_with_grid_spacing...
+ *                                        (device.hpp:47-60, 206-222)
+ *   rti_set_triangles                 -> geo::triangle_geometry buffers
+ *                                        (src/rti/geo/triangle_geometry.hpp:56-99)
+ *   rti_set_params                    -> device::set_number_of_rays, set_x,
+ *                                        set_y, set(i_direction&) and the
+ *                                        particle's sticking probability
+ *                                        (device.hpp:62-80;
+ *                                        src/rti/particle/i_particle.hpp:8-16)
+ *   rti_run                           -> device::run (device.hpp:82-107) =
+ *                                        geometry + boundary + source + tracer
+ *                                        + post-processing
+ *   rti_get_flux                      -> device::get_mc_estimates
+ *                                        (device.hpp:109-112, 153-187) and the
+ *                                        raw hit_accumulator::get_values
+ *                                        (src/rti/trace/hit_accumulator.hpp:140-142)
+ *   rti_get_counts                    -> device::get_hit_cnts (device.hpp:114-117)
+ *   rti_get_areas                     -> hit_accumulator::get_exposed_areas
+ *                                        (tracer.hpp:281-282, 385-413)
+ *   rti_get_neighbors                 -> point_cloud_disc_geometry::get_neighbors
+ *                                        (point_cloud_disc_geometry.hpp:391-394)
+ *   rti_get_stats                     -> trace::result (src/rti/trace/result.hpp:11-33)
+ *   rti_get_adjoint                   -> (no reference counterpart: d flux / ds
+ *                                        and d flux / dn of the same run)
+ *   rti_batch_*                       -> (no reference counterpart: many
+ *                                        geometries in one persistent launch)
+ *   rti_get_counters                  -> the tracer's hit / non-hit counters
+ *                                        (tracer.hpp:106-107, 186, 229)
+ *
+ * Error convention: every function returns RTI_OK (0) or an RTI_ERR_* code;
+ * rti_last_error(ctx) returns a message for the last failure on that ctx.
+ * One ctx per host thread; calls on one ctx must not overlap.
+ */
+#ifndef RTI_C_H
+#define RTI_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rti_ctx rti_ctx;
+
+enum rti_status {
+  RTI_OK = 0,
+  RTI_ERR_INVALID_ARG = 1,   /* null pointer, size mismatch, value out of range */
+  RTI_ERR_NO_GEOMETRY = 2,   /* run before set_discs / set_triangles, or N = 0  */
+  RTI_ERR_HIP = 3,           /* a HIP runtime call failed                        */
+  RTI_ERR_OVERFLOW = 4,      /* fixed-point accumulator could have overflowed    */
+  RTI_ERR_STATE = 5,         /* getter called before a successful run            */
+  RTI_ERR_ALLOC = 6          /* device allocation failed                         */
+};
+
+enum rti_bound_condition { RTI_BC_REFLECTIVE = 0, RTI_BC_PERIODIC = 1 };
+enum rti_kernel {
+  RTI_KERNEL_PERSISTENT = 0,  /* persistent, wavefront-scheduled (default)              */
+  RTI_KERNEL_PLAIN = 1,       /* one thread per ray, one launch per geometry            */
+  RTI_KERNEL_COUNT = 2        /* persistent with node / primitive / neighbour counters  */
+};
+enum rti_accum { RTI_ACCUM_FIXED64 = 0, RTI_ACCUM_FP64 = 1 };
+
+typedef struct rti_params {
+  uint64_t n_rays;        /* primary rays (reference default 1024, device.hpp:289)    */
+  uint32_t seed;          /* Philox key word 0                                        */
+  uint32_t instance;      /* Philox key word 1 (batch: added to the instance index)   */
+  int32_t bc_x, bc_y;     /* rti_bound_condition (reference default reflective)       */
+  float sticking;         /* sticking probability s in (0, 1]                          */
+  float cos_exponent;     /* source direction pdf ~ cos^n (1 = cosine, the default)    */
+  float diffuse_prob;     /* probability of a diffuse bounce; 1 = diffuse, 0 = mirror  */
+  float tnear;            /* ray epsilon, reference 1e-4 (tracer.hpp:179)              */
+  int32_t roulette;       /* 1 = rejection control 0.1/0.3 (rejection_control.hpp)    */
+  float src_z;            /* source plane height; NaN = derived from the geometry      */
+  int32_t kernel;         /* rti_kernel                                                 */
+  int32_t adjoint;        /* 1 = also accumulate d/ds and d/dn                          */
+  int32_t accum;          /* rti_accum                                                  */
+  int32_t block_size;     /* 0 = tuned default; else 64, 128, 192 or 256                */
+  int32_t lds_stack;      /* 0 = tuned default; else 4, 8, 16, 24 or 32 LDS stack slots */
+  int32_t waves_per_cu;   /* persistent kernel: resident waves per CU, 0 = default     */
+} rti_params;
+
+typedef struct rti_stats {
+  uint64_t n_prims;
+  uint64_t n_rays;
+  uint64_t segments;      /* closest-hit queries (primary, reflected, boundary, back face, miss) */
+  uint64_t surface_hits;  /* front-face primitive hits                                      */
+  uint64_t misses;        /* rays that left the domain                                       */
+  uint64_t truncated;     /* rays stopped by the per-ray segment cap                         */
+  uint64_t n_edges;       /* directed neighbour pairs                                        */
+  uint32_t bvh_depth;     /* deepest leaf of the hierarchy                                   */
+  int32_t fixed_shift;    /* fixed-point scale is 2^fixed_shift                              */
+  double trace_ms;        /* trace kernel(s) time (HIP events)                               */
+  double build_ms;        /* upload + LBVH + neighbours + areas                              */
+  double post_ms;         /* finalisation + download                                         */
+  float box[6];           /* boundary xmin, xmax, ymin, ymax, zmin, zmax                     */
+  float src_z;
+} rti_stats;
+
+/* One geometry of a batch. Exactly one of (xyzr, nrm) or (verts, tris) is set. */
+typedef struct rti_instance {
+  const float* xyzr;      /* n_prims x 4 */
+  const float* nrm;       /* n_prims x 3, unit length */
+  const float* verts;     /* n_verts x 3 */
+  uint64_t n_verts;
+  const uint32_t* tris;   /* n_prims x 3 */
+  uint64_t n_prims;
+  rti_params params;      /* per-instance rays, sticking, reflection, source, bcs; kernel knobs from rti_set_params */
+} rti_instance;
+
+void rti_default_params(rti_params* p);
+
+int rti_ctx_create(rti_ctx** out, int device);
+int rti_ctx_destroy(rti_ctx* ctx);
+const char* rti_last_error(const rti_ctx* ctx);
+
+int rti_set_discs(rti_ctx* ctx, const float* xyzr, const float* nrm, uint64_t n);
+int rti_set_triangles(rti_ctx* ctx, const float* verts, uint64_t n_verts, const uint32_t* tris, uint64_t n_tris);
+int rti_set_params(rti_ctx* ctx, const rti_params* p);
+
+/* Build (if the geometry changed), trace, finalise; results stay on the ctx. */
+int rti_run(rti_ctx* ctx);
+/* Split form for timing with inputs resident: rti_prepare builds the
+ * acceleration structures; rti_trace clears the accumulators and traces on
+ * the device without downloading; rti_finalize post-processes and downloads. */
+int rti_prepare(rti_ctx* ctx);
+int rti_trace(rti_ctx* ctx);
+int rti_finalize(rti_ctx* ctx);
+
+/* Any output pointer may be NULL. smoothed_norm: reference mc estimates
+ * (neighbourhood mean of value/area, divided by the max); raw: sum of
+ * deposits per primitive (reference `value`); sigma: Monte Carlo standard
+ * deviation of raw (per-deposit second moment). Arrays hold n_prims entries
+ * in input order. */
+int rti_get_flux(rti_ctx* ctx, float* smoothed_norm, double* raw, double* sigma);
+int rti_get_counts(rti_ctx* ctx, uint64_t* counts);
+int rti_get_areas(rti_ctx* ctx, float* areas);
+/* row_ptr: n_prims + 1 entries; col: up to cap entries (input order ids). */
+int rti_get_neighbors(rti_ctx* ctx, uint32_t* row_ptr, uint32_t* col, uint64_t cap, uint64_t* n_edges);
+int rti_get_stats(rti_ctx* ctx, rti_stats* out);
+/* d(raw)/ds and d(raw)/dn of the last run made with params.adjoint = 1. */
+int rti_get_adjoint(rti_ctx* ctx, double* draw_ds, double* draw_dn);
+/* Counters of the last trace: [0] segments, [1] surface hits, [2] misses,
+ * [3] truncated rays; after an RTI_KERNEL_COUNT run also [4] BVH node
+ * visits, [5] primitive tests, [6] neighbour tests, [7] deposits. */
+int rti_get_counters(rti_ctx* ctx, uint64_t out[8]);
+
+/* Batch: many geometries in one persistent launch (captured in a HIP graph
+ * and replayed by rti_batch_replay). Results per instance by index. */
+int rti_batch_set(rti_ctx* ctx, const rti_instance* inst, uint32_t n);
+int rti_batch_run(rti_ctx* ctx);
+int rti_batch_replay(rti_ctx* ctx, uint32_t times);
+int rti_batch_get_flux(rti_ctx* ctx, uint32_t i, float* smoothed_norm, double* raw, double* sigma);
+int rti_batch_get_counts(rti_ctx* ctx, uint32_t i, uint64_t* counts);
+int rti_batch_get_stats(rti_ctx* ctx, rti_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTI_C_H */

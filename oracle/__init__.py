@@ -1,0 +1,201 @@
+"""CPU oracle for the rti Monte Carlo flux path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg import this package, and only as the checker (or as the timed CPU
+baseline). The product (``librti_hip.so``, ``include/rti``, ``rti_amd``)
+never imports, links or calls it.
+
+``liboracle.so`` is built from ``rti_oracle.cpp`` by ``oracle/Makefile``; each
+function there cites the reference file:line it restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("prim_kind", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("n_prims", ctypes.c_uint64),
+        ("xyzr", _fp), ("nrm", _fp),
+        ("verts", _fp), ("n_verts", ctypes.c_uint64), ("tris", _u32p),
+        ("nb_ptr", _u32p), ("nb_col", _u32p),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n_rays", ctypes.c_uint64), ("ray_begin", ctypes.c_uint64),
+        ("seed", ctypes.c_uint32), ("instance", ctypes.c_uint32),
+        ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32),
+        ("sticking", ctypes.c_float), ("cos_exponent", ctypes.c_float),
+        ("diffuse_prob", ctypes.c_float), ("tnear", ctypes.c_float),
+        ("roulette", ctypes.c_int32), ("fp64", ctypes.c_int32),
+        ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float),
+        ("threads", ctypes.c_int32), ("adjoint", ctypes.c_int32),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("s1", _dp), ("s2", _dp), ("cnt", _u64p), ("dsum_ds", _dp), ("dsum_dn", _dp),
+        ("segments", ctypes.c_uint64), ("surface_hits", ctypes.c_uint64),
+        ("misses", ctypes.c_uint64), ("seconds", ctypes.c_double),
+    ]
+
+
+def _declare(L):
+    L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+    L.orc_local_intersect.argtypes = [_fp, _fp, _fp, _fp]
+    L.orc_local_intersect.restype = ctypes.c_int
+    L.orc_area_inside.argtypes = [ctypes.c_float] * 4 + [_fp, _fp]
+    L.orc_area_inside.restype = ctypes.c_float
+    L.orc_neighbors.argtypes = [_fp, ctypes.c_uint64, _u32p, _u32p]
+    L.orc_neighbors.restype = ctypes.c_int64
+    L.orc_smooth_normalize.argtypes = [_dp, _fp, _u32p, _u32p, ctypes.c_uint64, _fp]
+    L.orc_trace.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.POINTER(Result)]
+    L.orc_trace.restype = ctypes.c_int
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c, _u32p), _ptr(k, _u32p), _ptr(out, _u32p))
+    return out
+
+
+def local_intersect(org, dirn, disc, nrm):
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    o, d, c, n = f(org), f(dirn), f(disc), f(nrm)
+    return bool(lib().orc_local_intersect(_ptr(o, _fp), _ptr(d, _fp), _ptr(c, _fp), _ptr(n, _fp)))
+
+
+def area_inside(box, disc, nrm):
+    """box = (xmin, ymin, xmax, ymax)."""
+    c = np.ascontiguousarray(disc, dtype=np.float32)
+    n = np.ascontiguousarray(nrm, dtype=np.float32)
+    return float(lib().orc_area_inside(*[ctypes.c_float(v) for v in box], _ptr(c, _fp), _ptr(n, _fp)))
+
+
+def neighbors(xyzr):
+    """Overlap graph (CSR, columns ascending) with the reference's predicate."""
+    x = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+    n = x.shape[0]
+    row = np.zeros(n + 1, dtype=np.uint32)
+    e = lib().orc_neighbors(_ptr(x, _fp), n, _ptr(row, _u32p), None)
+    col = np.zeros(max(int(e), 1), dtype=np.uint32)
+    lib().orc_neighbors(_ptr(x, _fp), n, _ptr(row, _u32p), _ptr(col, _u32p))
+    return row, col[: int(e)]
+
+
+def smooth_normalize(values, areas, row, col):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    a = np.ascontiguousarray(areas, dtype=np.float32)
+    out = np.zeros(v.shape[0], dtype=np.float32)
+    r = np.ascontiguousarray(row, dtype=np.uint32) if row is not None else None
+    c = np.ascontiguousarray(col, dtype=np.uint32) if col is not None and len(col) else np.zeros(1, np.uint32)
+    lib().orc_smooth_normalize(_ptr(v, _dp), _ptr(a, _fp), _ptr(r, _u32p), _ptr(c, _u32p) if r is not None else None,
+                               v.shape[0], _ptr(out, _fp))
+    return out
+
+
+def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc_y=0, cos_exponent=1.0,
+          diffuse_prob=1.0, tnear=1e-4, roulette=True, fp64=False, threads=0, ray_begin=0, adjoint=False,
+          neighbors=None):
+    """Run the oracle trace. ``scene`` is a dict from ``rti_amd.geometry``.
+
+    Returns a dict with s1, s2, cnt (per primitive) and the counters.
+    """
+    keep = []
+    sc = Scene()
+    if scene["kind"] == "discs":
+        xyzr = np.ascontiguousarray(scene["xyzr"], dtype=np.float32)
+        nrm = np.ascontiguousarray(scene["normals"], dtype=np.float32)
+        keep += [xyzr, nrm]
+        n = xyzr.shape[0]
+        sc.prim_kind = 0
+        sc.xyzr = _ptr(xyzr, _fp)
+        sc.nrm = _ptr(nrm, _fp)
+        if neighbors is None:
+            neighbors = globals()["neighbors"](xyzr)
+        row, col = neighbors
+        row = np.ascontiguousarray(row, dtype=np.uint32)
+        col = np.ascontiguousarray(col if len(col) else np.zeros(1), dtype=np.uint32)
+        keep += [row, col]
+        sc.nb_ptr = _ptr(row, _u32p)
+        sc.nb_col = _ptr(col, _u32p)
+    else:
+        verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
+        tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
+        keep += [verts, tris]
+        n = tris.shape[0]
+        sc.prim_kind = 1
+        sc.verts = _ptr(verts, _fp)
+        sc.n_verts = verts.shape[0]
+        sc.tris = _ptr(tris, _u32p)
+    sc.n_prims = n
+    p = Params()
+    p.n_rays = int(n_rays)
+    p.ray_begin = int(ray_begin)
+    p.seed = int(seed)
+    p.instance = int(instance)
+    p.bc_x, p.bc_y = int(bc_x), int(bc_y)
+    p.sticking = float(sticking)
+    p.cos_exponent = float(cos_exponent)
+    p.diffuse_prob = float(diffuse_prob)
+    p.tnear = float(tnear)
+    p.roulette = 1 if roulette else 0
+    p.fp64 = 1 if fp64 else 0
+    for i in range(6):
+        p.box[i] = float(box[i])
+    p.src_z = float(src_z)
+    p.threads = int(threads)
+    p.adjoint = 1 if adjoint else 0
+    out = {"s1": np.zeros(n), "s2": np.zeros(n), "cnt": np.zeros(n, dtype=np.uint64)}
+    if adjoint:
+        out["ds"] = np.zeros(n)
+        out["dn"] = np.zeros(n)
+    r = Result()
+    r.s1 = _ptr(out["s1"], _dp)
+    r.s2 = _ptr(out["s2"], _dp)
+    r.cnt = _ptr(out["cnt"], _u64p)
+    if adjoint:
+        r.dsum_ds = _ptr(out["ds"], _dp)
+        r.dsum_dn = _ptr(out["dn"], _dp)
+    rc = lib().orc_trace(ctypes.byref(sc), ctypes.byref(p), ctypes.byref(r))
+    if rc != 0:
+        raise RuntimeError(f"orc_trace failed with code {rc}")
+    out.update(segments=int(r.segments), surface_hits=int(r.surface_hits), misses=int(r.misses),
+               seconds=float(r.seconds))
+    return out

@@ -1,0 +1,643 @@
+// Acceleration-structure build for gfx950: Morton codes, an LSD radix sort,
+// the Karras (HPG 2012) binary radix tree, an atomic-flag bottom-up refit,
+// the disc-overlap graph (CSR) queried through that tree, and the exposed
+// disc areas.
+//
+// Replaces the Embree BVH build (reference src/rti/trace/tracer.hpp:74-100),
+// disc_neighborhood::setup_neighborhood (src/rti/geo/disc_neighborhood.hpp:20-220)
+// and tracer::compute_disc_areas (tracer.hpp:385-413).
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "internal.hpp"
+
+using namespace rti_dev;
+
+namespace rti_impl {
+namespace {
+
+constexpr int kScanBlock = 1024;
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (uint32) -- three passes: per-block totals, scan of the
+// totals by one block, per-block scan plus offset.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, uint32_t& total) {
+  // sh: blockDim.x / 64 words
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int w = 0; w < nw; ++w) {
+      uint32_t t = sh[w];
+      sh[w] = acc;
+      acc += t;
+    }
+    sh[nw] = acc;
+  }
+  __syncthreads();
+  uint32_t r = x - v + sh[wid];
+  total = sh[nw];
+  __syncthreads();
+  return r;
+}
+
+__global__ void k_scan_totals(const uint32_t* in, uint32_t n, uint32_t* totals) {
+  __shared__ uint32_t sh[kScanBlock / 64 + 1];
+  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t v = i < n ? in[i] : 0u;
+  uint32_t tot;
+  block_scan_excl(v, sh, tot);
+  if (threadIdx.x == 0) totals[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_single(uint32_t* data, uint32_t n) {
+  __shared__ uint32_t sh[kScanBlock / 64 + 1];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kScanBlock) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < n ? data[i] : 0u;
+    uint32_t tot;
+    uint32_t r = block_scan_excl(v, sh, tot);
+    if (i < n) data[i] = r + carry;
+    carry += tot;
+  }
+}
+
+__global__ void k_scan_apply(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_t* offsets) {
+  __shared__ uint32_t sh[kScanBlock / 64 + 1];
+  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t v = i < n ? in[i] : 0u;
+  uint32_t tot;
+  uint32_t r = block_scan_excl(v, sh, tot);
+  if (i < n) out[i] = r + offsets[blockIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (30-bit key, uint32 value), 8 bits per pass, stable.
+// ---------------------------------------------------------------------------
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr uint32_t kRsTile = kRsThreads * kRsItems;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, uint32_t n, int shift,
+                                                        uint32_t* hist, uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t base = blockIdx.x * kRsTile;
+  for (int it = 0; it < kRsItems; ++it) {
+    uint32_t i = base + it * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin,
+                                                           uint32_t* kout, uint32_t* vout, uint32_t n, int shift,
+                                                           const uint32_t* offs, uint32_t nblocks) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[kRsThreads / 64][257];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  base[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t tile = blockIdx.x * kRsTile;
+  for (int it = 0; it < kRsItems; ++it) {
+    for (int q = threadIdx.x; q < (kRsThreads / 64) * 257; q += kRsThreads) (&wcnt[0][0])[q] = 0;
+    __syncthreads();
+    uint32_t i = tile + it * kRsThreads + threadIdx.x;
+    bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t dig = valid ? ((key >> shift) & 255u) : 256u;
+    // lanes of this wave holding the same digit (invalid lanes form their own group)
+    unsigned long long same = __ballot(valid) ;
+    if (!valid) same = ~same;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      unsigned long long bal = __ballot((dig >> b) & 1u);
+      same &= ((dig >> b) & 1u) ? bal : ~bal;
+    }
+    uint32_t rank = __popcll(same & lt);
+    if (rank == 0) wcnt[wid][dig] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = base[dig] + rank;
+      for (int w = 0; w < wid; ++w) off += wcnt[w][dig];
+      kout[off] = key;
+      vout[off] = val;
+    }
+    __syncthreads();
+    {
+      uint32_t add = 0;
+      for (int w = 0; w < kRsThreads / 64; ++w) add += wcnt[w][threadIdx.x];
+      base[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Primitive setup: Morton code of the centroid.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+struct setup_args {
+  int kind;
+  uint32_t n;
+  const float* xyzr;
+  const float* nrm;
+  const float* verts;
+  const uint32_t* tris;
+  float cmin0, cmin1, cmin2, cinv0, cinv1, cinv2;
+};
+
+__device__ __forceinline__ void prim_centroid(const setup_args& a, uint32_t i, float& x, float& y, float& z) {
+  if (a.kind == 0) {
+    x = a.xyzr[4 * i];
+    y = a.xyzr[4 * i + 1];
+    z = a.xyzr[4 * i + 2];
+  } else {
+    const uint32_t* t = a.tris + 3 * (size_t)i;
+    const float* v0 = a.verts + 3 * (size_t)t[0];
+    const float* v1 = a.verts + 3 * (size_t)t[1];
+    const float* v2 = a.verts + 3 * (size_t)t[2];
+    x = (v0[0] + v1[0] + v2[0]) * (1.0f / 3.0f);
+    y = (v0[1] + v1[1] + v2[1]) * (1.0f / 3.0f);
+    z = (v0[2] + v1[2] + v2[2]) * (1.0f / 3.0f);
+  }
+}
+
+__global__ void k_morton(setup_args a, uint32_t* keys, uint32_t* vals) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float x, y, z;
+  prim_centroid(a, i, x, y, z);
+  auto q = [](float v, float lo, float inv) {
+    float t = (v - lo) * inv;
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return (uint32_t)fminf(t * 1024.0f, 1023.0f);
+  };
+  uint32_t code = (spread10(q(x, a.cmin0, a.cinv0)) << 2) | (spread10(q(y, a.cmin1, a.cinv1)) << 1) |
+                  spread10(q(z, a.cmin2, a.cinv2));
+  keys[i] = code;
+  vals[i] = i;
+}
+
+// Stored primitive records in Morton order, the permutation, and leaf boxes.
+__global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t prim_begin, float4* pa, float4* pb,
+                                float4* pc, uint32_t* perm, float4* leaf_lo, float4* leaf_hi) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  uint32_t i = order[j];
+  size_t g = (size_t)prim_begin + j;
+  perm[g] = i;
+  float lo[3], hi[3];
+  if (a.kind == 0) {
+    float4 c = make_float4(a.xyzr[4 * (size_t)i], a.xyzr[4 * (size_t)i + 1], a.xyzr[4 * (size_t)i + 2],
+                           a.xyzr[4 * (size_t)i + 3]);
+    float n0 = a.nrm[3 * (size_t)i], n1 = a.nrm[3 * (size_t)i + 1], n2 = a.nrm[3 * (size_t)i + 2];
+    pa[g] = c;
+    pb[g] = make_float4(n0, n1, n2, 0.0f);
+    pc[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float nn[3] = {n0, n1, n2};
+    float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // oriented disc extent along axis k: r * sqrt(1 - n_k^2), padded
+      float ext = c.w * sqrtf(fmaxf(0.0f, 1.0f - nn[k] * nn[k])) + c.w * 1e-5f;
+      lo[k] = cc[k] - ext;
+      hi[k] = cc[k] + ext;
+    }
+  } else {
+    const uint32_t* t = a.tris + 3 * (size_t)i;
+    float v[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[q][k] = a.verts[3 * (size_t)t[q] + k];
+    pa[g] = make_float4(v[0][0], v[0][1], v[0][2], -1.0f);  // w < 0 marks a triangle
+    pb[g] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2], 0.0f);
+    pc[g] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2], 0.0f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float l = fminf(v[0][k], fminf(v[1][k], v[2][k]));
+      float h = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
+      float pad = 1e-6f * fmaxf(1.0f, fmaxf(fabsf(l), fabsf(h)));
+      lo[k] = l - pad;
+      hi[k] = h + pad;
+    }
+  }
+  leaf_lo[j] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+  leaf_hi[j] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Karras radix tree over the sorted codes (ties broken by sorted position).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int delta(const uint32_t* codes, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = codes[i], b = codes[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+__global__ void k_karras(const uint32_t* codes, int n, uint32_t prim_begin, uint32_t node_begin, bvh_node* nodes,
+                         int32_t* parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = delta(codes, n, i, i - d);
+  int lmax = 2;
+  while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = delta(codes, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int gamma = i + s * d + min(d, 0);
+  int lo = min(i, j), hi = max(i, j);
+  int c0, c1;
+  // parent[] local ids: internal nodes 0..n-2, leaves n-1 .. 2n-2
+  if (lo == gamma) {
+    c0 = ~(int)(prim_begin + gamma);
+    parent[(n - 1) + gamma] = i;
+  } else {
+    c0 = (int)(node_begin + gamma);
+    parent[gamma] = i;
+  }
+  if (hi == gamma + 1) {
+    c1 = ~(int)(prim_begin + gamma + 1);
+    parent[(n - 1) + gamma + 1] = i;
+  } else {
+    c1 = (int)(node_begin + gamma + 1);
+    parent[gamma + 1] = i;
+  }
+  nodes[node_begin + i].child = make_int4(c0, c1, 0, 0);
+  if (i == 0) parent[0] = -1;
+}
+
+__device__ __forceinline__ void write_slot(bvh_node* nd, int slot, float4 lo, float4 hi) {
+  if (slot == 0) {
+    nd->xy0 = make_float4(lo.x, hi.x, lo.y, hi.y);
+    nd->z01.x = lo.z;
+    nd->z01.y = hi.z;
+  } else {
+    nd->xy1 = make_float4(lo.x, hi.x, lo.y, hi.y);
+    nd->z01.z = lo.z;
+    nd->z01.w = hi.z;
+  }
+}
+
+// Bottom-up refit: each leaf writes its box into its parent's slot; the
+// second arrival at a node (acq_rel counter) merges both slots and climbs.
+// Also records the deepest leaf (number of internal ancestors + 1).
+__global__ void k_refit(int n, uint32_t prim_begin, uint32_t node_begin, bvh_node* nodes, const int32_t* parent,
+                        uint32_t* flags, const float4* leaf_lo, const float4* leaf_hi, uint32_t* max_depth) {
+  int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= n) return;
+  float4 lo = leaf_lo[leaf], hi = leaf_hi[leaf];
+  int node = parent[(n - 1) + leaf];
+  uint32_t depth = 1;
+  for (int a = node; a >= 0; a = parent[a]) ++depth;
+  atomicMax(max_depth, depth);
+  int code = ~(int)(prim_begin + leaf);  // how the parent refers to where we come from
+  while (node >= 0) {
+    bvh_node* nd = nodes + node_begin + node;
+    int4 ch = nd->child;
+    write_slot(nd, ch.x == code ? 0 : 1, lo, hi);
+    unsigned prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0) return;  // the sibling subtree is not finished; it will continue from here
+    float4 a = nd->xy0, b = nd->xy1, z = nd->z01;
+    lo = make_float4(fminf(a.x, b.x), fminf(a.z, b.z), fminf(z.x, z.z), 0.0f);
+    hi = make_float4(fmaxf(a.y, b.y), fmaxf(a.w, b.w), fmaxf(z.y, z.w), 0.0f);
+    code = (int)(node_begin + node);
+    node = parent[node];
+  }
+}
+
+// One primitive: a root whose two children are that leaf.
+__global__ void k_single_root(uint32_t prim_begin, uint32_t node_begin, bvh_node* nodes, const float4* leaf_lo,
+                              const float4* leaf_hi, uint32_t* max_depth) {
+  bvh_node* nd = nodes + node_begin;
+  write_slot(nd, 0, leaf_lo[0], leaf_hi[0]);
+  write_slot(nd, 1, leaf_lo[0], leaf_hi[0]);
+  nd->child = make_int4(~(int)prim_begin, ~(int)prim_begin, 0, 0);
+  *max_depth = 2;
+}
+
+// ---------------------------------------------------------------------------
+// Disc-overlap graph. The pair predicate restates disc_neighborhood::check_dist
+// (src/rti/geo/disc_neighborhood.hpp:190-220) in FP32 without contraction:
+// per-axis rejection at >= r_i + r_j, then sqrt(dx^2 + dy^2 + dz^2) < r_i + r_j.
+// Candidates come from the instance's tree with the query box
+// c_i +- (r_i + r_max), which contains every disc that can overlap disc i.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool overlap_pred(float4 p1, float4 p2) {
+#pragma clang fp contract(off)
+  float sr = p1.w + p2.w;
+  if (fabsf(p2.x - p1.x) >= sr) return false;
+  if (fabsf(p1.y - p2.y) >= sr) return false;
+  if (fabsf(p1.z - p2.z) >= sr) return false;
+  float d1 = p1.x - p2.x, d2 = p1.y - p2.y, d3 = p1.z - p2.z;
+  float dist = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(d1, d1), __fmul_rn(d2, d2)), __fmul_rn(d3, d3)));
+  return dist < p1.w + p2.w;
+}
+
+__device__ __forceinline__ bool box_overlap(float lox, float hix, float loy, float hiy, float loz, float hiz,
+                                            float3 qlo, float3 qhi) {
+  return !(hix < qlo.x || lox > qhi.x || hiy < qlo.y || loy > qhi.y || hiz < qlo.z || loz > qhi.z);
+}
+
+template <bool kWrite>
+__global__ void k_neighbors(const float4* pa, uint32_t prim_begin, uint32_t n, uint32_t root, const bvh_node* nodes,
+                            float rmax, uint32_t* counts, const uint32_t* row_ptr, uint32_t* col) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t self = prim_begin + j;
+  float4 p = pa[self];
+  float reach = p.w + rmax;
+  float3 qlo = make_float3(p.x - reach, p.y - reach, p.z - reach);
+  float3 qhi = make_float3(p.x + reach, p.y + reach, p.z + reach);
+  uint32_t cnt = 0;
+  uint32_t w = kWrite ? row_ptr[self] : 0u;
+  int stack[64];
+  int sp = 0;
+  int cur = (int)root;
+  for (;;) {
+    const bvh_node& nd = nodes[cur];
+    int4 ch = nd.child;
+    float4 a = nd.xy0, b = nd.xy1, z = nd.z01;
+    bool h0 = box_overlap(a.x, a.y, a.z, a.w, z.x, z.y, qlo, qhi);
+    bool h1 = box_overlap(b.x, b.y, b.z, b.w, z.z, z.w, qlo, qhi);
+    if (ch.x == ch.y) h1 = false;  // single-primitive root
+    int next = -1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      bool h = c == 0 ? h0 : h1;
+      int cc = c == 0 ? ch.x : ch.y;
+      if (!h) continue;
+      if (cc < 0) {
+        uint32_t q = (uint32_t)(~cc);
+        if (q != self && overlap_pred(p, pa[q])) {
+          if (kWrite) col[w] = q;
+          ++w;
+          ++cnt;
+        }
+      } else if (next < 0) {
+        next = cc;
+      } else {
+        stack[sp++] = cc;
+      }
+    }
+    if (next >= 0) {
+      cur = next;
+    } else if (sp > 0) {
+      cur = stack[--sp];
+    } else {
+      break;
+    }
+  }
+  if (!kWrite) counts[j] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Exposed disc area inside the x-y box. Restates
+// disc_bounding_box_intersector::area_inside
+// (src/rti/geo/disc_bounding_box_intersector.hpp:42-76; closest approach
+// :372-441; face segments :116-125; adjacent-face corner overlap :127-261).
+// Faces in the reference's order: right, bottom, left, top (:349-356).
+// Triangles: full triangle area.
+// ---------------------------------------------------------------------------
+__device__ float face_approach(int f, float xmin, float ymin, float xmax, float ymax, float cx, float cy, float r,
+                               float nx, float ny, float nz) {
+  float pos, plane, nperp;
+  if (f == 0) { pos = cx; plane = xmax; nperp = ny; }
+  else if (f == 1) { pos = -cy; plane = -ymin; nperp = nx; }
+  else if (f == 2) { pos = -cx; plane = -xmin; nperp = ny; }
+  else { pos = cy; plane = ymax; nperp = nx; }
+  float xterm = r * sqrtf(nz * nz + nperp * nperp);
+  if (pos + xterm <= plane) return FLT_MAX;
+  if (pos - xterm >= plane) return -FLT_MAX;
+  if (xterm <= 1e-9f) return FLT_MAX;
+  return (plane - pos) * r / xterm;
+}
+
+__device__ float disc_area_inside(float xmin, float ymin, float xmax, float ymax, float4 disc, float4 nin) {
+#pragma clang fp contract(off)
+  float r = disc.w;
+  float nl = sqrtf(nin.x * nin.x + nin.y * nin.y + nin.z * nin.z);
+  f3 n = make(nin.x / nl, nin.y / nl, nin.z / nl);
+  if (r <= 0.0f) return 0.0f;
+  float full = r * r * 3.14159265358979323846f;
+  float cx = disc.x, cy = disc.y;
+  if ((xmin <= cx - r && cx + r <= xmax) && (ymin <= cy - r && cy + r <= ymax)) return full;
+  if ((cx + r <= xmin || xmax <= cx - r) || (cy + r <= ymin || ymax <= cy - r)) return 0.0f;
+  float a[4];
+  for (int f = 0; f < 4; ++f) {
+    a[f] = face_approach(f, xmin, ymin, xmax, ymax, cx, cy, r, n.x, n.y, n.z);
+    if (a[f] < -r) return 0.0f;
+  }
+  const float fnx[4] = {-1.0f, 0.0f, 1.0f, 0.0f}, fny[4] = {0.0f, 1.0f, 0.0f, -1.0f};
+  const float fcx[4] = {xmax, xmax, xmin, xmin}, fcy[4] = {ymax, ymin, ymin, ymax};
+  float area = 0.0f;
+  for (int f = 0; f < 4; ++f) {
+    if (-r < a[f] && a[f] < r) {
+      double ang = 2.0 * acos((double)a[f] / r);
+      double seg = (double)(r * r / 2.0f) * (ang - sin(ang));
+      area = (float)((double)area + seg);
+    }
+  }
+  f3 cp = make(disc.x, disc.y, disc.z);
+  for (int f = 0; f < 4; ++f) {
+    int g = (f + 1) & 3;
+    float d1 = a[f], d2 = a[g];
+    if (!(-r < d1 && d1 < r && -r < d2 && d2 < r)) continue;
+    f3 n1 = make(fnx[f], fny[f], 0.0f), n2 = make(fnx[g], fny[g], 0.0f);
+    f3 i1 = normalize(cross(n, n1));
+    f3 i2 = normalize(cross(n, n2));
+    if (dot(i1, n2) >= 0.0f) i1 = i1 * -1.0f;
+    if (dot(i2, n1) >= 0.0f) i2 = i2 * -1.0f;
+    float xi = fcx[g], yi = fcy[g];
+    float zi = (n.x * cp.x + n.y * cp.y + n.z * cp.z - n.x * xi - n.y * yi) / n.z;
+    f3 ip = make(xi, yi, zi);
+    f3 dc = cp - ip;
+    if (sqrtf(dot(dc, dc)) >= r) continue;
+    f3 q1 = ip + i1 * dot(dc, i1) + i1 * sqrtf(r * r - d1 * d1);
+    f3 q2 = ip + i2 * dot(dc, i2) + i2 * sqrtf(r * r - d2 * d2);
+    f3 e1 = q1 - cp, e2 = q2 - cp;
+    float ang = acosf(dot(e1, e2) / sqrtf(dot(e1, e1)) / sqrtf(dot(e2, e2)));
+    float seg = r * r / 2.0f * (ang - sinf(ang));
+    f3 cr = cross(q1 - ip, q2 - ip);
+    float tri = 0.5f * sqrtf(dot(cr, cr));
+    area -= seg + tri;
+  }
+  return full - area;
+}
+
+__global__ void k_areas(int kind, const float4* pa, const float4* pb, const float4* pc, uint32_t prim_begin,
+                        uint32_t n, float xmin, float ymin, float xmax, float ymax, float* areas) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  size_t g = (size_t)prim_begin + j;
+  float A;
+  if (kind == 0) {
+    A = disc_area_inside(xmin, ymin, xmax, ymax, pa[g], pb[g]);
+  } else {
+    float4 e1 = pb[g], e2 = pc[g];
+    f3 c = cross(make(e1.x, e1.y, e1.z), make(e2.x, e2.y, e2.z));
+    A = 0.5f * sqrtf(dot(c, c));
+  }
+  areas[g] = A;
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+size_t scan_tmp_size(uint32_t n) { return (size_t)(n + kScanBlock - 1) / kScanBlock + 1; }
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  k_scan_totals<<<nb, kScanBlock, 0, stream>>>(in, n, tmp);
+  k_scan_single<<<1, kScanBlock, 0, stream>>>(tmp, nb);
+  k_scan_apply<<<nb, kScanBlock, 0, stream>>>(in, out, n, tmp);
+  return hipGetLastError();
+}
+
+hipError_t scratch_reserve(build_scratch& s, uint32_t n) {
+  if (n <= s.cap && s.keys[0]) return hipSuccess;
+  scratch_free(s);
+  uint32_t cap = n < 1024 ? 1024 : n;
+  size_t nblocks = (cap + kRsTile - 1) / kRsTile;
+  hipError_t e;
+#define RTI_ALLOC(ptr, bytes)                              \
+  e = hipMalloc((void**)&(ptr), (bytes));                 \
+  if (e != hipSuccess) { scratch_free(s); return e; }
+  RTI_ALLOC(s.keys[0], cap * 4ull);
+  RTI_ALLOC(s.keys[1], cap * 4ull);
+  RTI_ALLOC(s.vals[0], cap * 4ull);
+  RTI_ALLOC(s.vals[1], cap * 4ull);
+  RTI_ALLOC(s.hist, 256ull * nblocks * 4ull);
+  size_t st = scan_tmp_size((uint32_t)(256 * nblocks));
+  size_t st2 = scan_tmp_size(cap + 1);
+  RTI_ALLOC(s.scan_tmp, (st > st2 ? st : st2) * 4ull);
+  RTI_ALLOC(s.parent, (2ull * cap) * 4ull);
+  RTI_ALLOC(s.flags, cap * 4ull);
+  RTI_ALLOC(s.leaf_lo, cap * 16ull);
+  RTI_ALLOC(s.leaf_hi, cap * 16ull);
+  RTI_ALLOC(s.counts, (cap + 1ull) * 4ull);
+  RTI_ALLOC(s.depth, 4ull);
+#undef RTI_ALLOC
+  s.cap = cap;
+  return hipSuccess;
+}
+
+void scratch_free(build_scratch& s) {
+  void* ptrs[] = {s.keys[0], s.keys[1], s.vals[0], s.vals[1], s.hist, s.scan_tmp, s.parent,
+                  s.flags, s.leaf_lo, s.leaf_hi, s.counts, s.depth};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  s = build_scratch{};
+}
+
+hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* depth_out,
+                          hipStream_t stream) {
+  const uint32_t n = job.n;
+  if (n == 0) return hipErrorInvalidValue;
+  hipError_t e = scratch_reserve(s, n);
+  if (e != hipSuccess) return e;
+  setup_args a;
+  a.kind = job.prim_kind;
+  a.n = n;
+  a.xyzr = job.d_xyzr;
+  a.nrm = job.d_nrm;
+  a.verts = job.d_verts;
+  a.tris = job.d_tris;
+  float inv[3];
+  for (int k = 0; k < 3; ++k) {
+    float ext = job.cmax[k] - job.cmin[k];
+    inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+  }
+  a.cmin0 = job.cmin[0]; a.cmin1 = job.cmin[1]; a.cmin2 = job.cmin[2];
+  a.cinv0 = inv[0]; a.cinv1 = inv[1]; a.cinv2 = inv[2];
+  const unsigned T = 256;
+  k_morton<<<blocks_for(n, T), T, 0, stream>>>(a, s.keys[0], s.vals[0]);
+  // 30-bit keys: four 8-bit passes, result back in buffer 0
+  uint32_t nblocks = (n + kRsTile - 1) / kRsTile;
+  int cur = 0;
+  for (int shift = 0; shift < 32; shift += 8) {
+    k_rs_hist<<<nblocks, kRsThreads, 0, stream>>>(s.keys[cur], n, shift, s.hist, nblocks);
+    e = exclusive_scan_u32(s.hist, s.hist, 256u * nblocks, s.scan_tmp, stream);
+    if (e != hipSuccess) return e;
+    k_rs_scatter<<<nblocks, kRsThreads, 0, stream>>>(s.keys[cur], s.vals[cur], s.keys[cur ^ 1], s.vals[cur ^ 1], n,
+                                                     shift, s.hist, nblocks);
+    cur ^= 1;
+  }
+  k_scatter_prims<<<blocks_for(n, T), T, 0, stream>>>(a, s.vals[cur], job.prim_begin, st.prim_a, st.prim_b,
+                                                       st.prim_c, st.perm, s.leaf_lo, s.leaf_hi);
+  e = hipMemsetAsync(s.depth, 0, 4, stream);
+  if (e != hipSuccess) return e;
+  if (n == 1) {
+    k_single_root<<<1, 1, 0, stream>>>(job.prim_begin, job.node_begin, st.nodes, s.leaf_lo, s.leaf_hi, s.depth);
+  } else {
+    k_karras<<<blocks_for(n - 1, T), T, 0, stream>>>(s.keys[cur], (int)n, job.prim_begin, job.node_begin, st.nodes,
+                                                     s.parent);
+    e = hipMemsetAsync(s.flags, 0, (n - 1) * 4ull, stream);
+    if (e != hipSuccess) return e;
+    k_refit<<<blocks_for(n, T), T, 0, stream>>>((int)n, job.prim_begin, job.node_begin, st.nodes, s.parent, s.flags,
+                                                s.leaf_lo, s.leaf_hi, s.depth);
+  }
+  k_areas<<<blocks_for(n, T), T, 0, stream>>>(job.prim_kind, st.prim_a, st.prim_b, st.prim_c, job.prim_begin, n,
+                                              job.area_box[0], job.area_box[1], job.area_box[2], job.area_box[3],
+                                              st.areas);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(depth_out, s.depth, 4, hipMemcpyDeviceToHost, stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(stream);
+}
+
+hipError_t neighbor_count(const build_job& job, const store_ptrs& st, build_scratch& s, hipStream_t stream) {
+  const unsigned T = 128;
+  k_neighbors<false><<<blocks_for(job.n, T), T, 0, stream>>>(st.prim_a, job.prim_begin, job.n, job.node_begin,
+                                                              st.nodes, job.rmax, s.counts, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t neighbor_fill(const build_job& job, const store_ptrs& st, hipStream_t stream) {
+  const unsigned T = 128;
+  k_neighbors<true><<<blocks_for(job.n, T), T, 0, stream>>>(st.prim_a, job.prim_begin, job.n, job.node_begin,
+                                                             st.nodes, job.rmax, nullptr, st.nb_ptr, st.nb_col);
+  return hipGetLastError();
+}
+
+}  // namespace rti_impl

@@ -1,0 +1,122 @@
+// Host-side declarations shared by the translation units of librti_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rti_device.hpp"
+
+namespace rti_impl {
+
+// Device buffers of one instance's build inputs (input order) and where its
+// outputs go inside the scene store.
+struct build_job {
+  int prim_kind;            // 0 discs, 1 triangles
+  uint32_t n;               // primitives
+  uint32_t prim_begin;      // first absolute prim slot
+  uint32_t node_begin;      // first absolute node slot (max(n - 1, 1) nodes)
+  const float* d_xyzr;      // discs, input order, n x 4
+  const float* d_nrm;       // discs, n x 3
+  const float* d_verts;     // triangles
+  const uint32_t* d_tris;   // triangles, n x 3
+  float cmin[3], cmax[3];   // centroid bounds (for Morton quantisation)
+  float rmax;               // largest disc radius (neighbour query reach)
+  float area_box[4];        // xmin, ymin, xmax, ymax for the exposed area
+};
+
+// Scene store (absolute indexing; see rti_device.hpp).
+struct store_ptrs {
+  float4* prim_a;
+  float4* prim_b;
+  float4* prim_c;
+  uint32_t* perm;           // stored position -> input id (per instance)
+  rti_dev::bvh_node* nodes;
+  uint32_t* nb_ptr;         // n_total + 1
+  uint32_t* nb_col;
+  float* areas;             // per stored position
+};
+
+// Scratch for builds (sized for the largest instance).
+struct build_scratch {
+  uint32_t cap = 0;
+  uint32_t* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t* hist = nullptr;       // 256 x blocks
+  uint32_t* scan_tmp = nullptr;
+  int32_t* parent = nullptr;      // 2n - 1: internal nodes then leaves (local ids)
+  uint32_t* flags = nullptr;      // n - 1
+  float4* leaf_lo = nullptr;      // n
+  float4* leaf_hi = nullptr;      // n
+  uint32_t* counts = nullptr;     // n + 1 (neighbour degrees)
+  uint32_t* depth = nullptr;      // 1
+};
+
+hipError_t scratch_reserve(build_scratch& s, uint32_t n);
+void scratch_free(build_scratch& s);
+
+// Builds prims (Morton order), perm, BVH nodes and areas for one instance.
+// Neighbour CSR is built separately because its size is data dependent.
+hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* depth_out,
+                          hipStream_t stream);
+// Degrees of the overlap graph (discs), into s.counts[0..n).
+hipError_t neighbor_count(const build_job& job, const store_ptrs& st, build_scratch& s, hipStream_t stream);
+// Columns, with st.nb_ptr already holding absolute row offsets.
+hipError_t neighbor_fill(const build_job& job, const store_ptrs& st, hipStream_t stream);
+// Exclusive scan of n uint32 values (out may alias in). tmp: scan_tmp_size(n) words.
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t stream);
+size_t scan_tmp_size(uint32_t n);
+
+// ---- trace ----
+struct trace_launch {
+  const rti_dev::inst_desc* desc;   // n_inst descriptors
+  uint32_t n_inst;
+  uint64_t ray_lo, ray_hi;          // global ray ids [ray_lo, ray_hi) traced by this launch
+  const float4* prim_a;
+  const float4* prim_b;
+  const float4* prim_c;
+  const rti_dev::bvh_node* nodes;
+  const uint32_t* nb_ptr;
+  const uint32_t* nb_col;
+  unsigned long long* acc_s1;       // fixed point (or double bits in FP64 mode)
+  unsigned long long* acc_s2;
+  unsigned long long* acc_cnt;
+  unsigned long long* acc_ds;       // adjoint, may be null
+  unsigned long long* acc_dn;
+  unsigned long long* queue;        // [0]: next ray id
+  unsigned long long* stats;        // segments, surface hits, misses, truncated
+  double fx_scale;                  // 2^shift
+  int adjoint;
+  int accum_fp64;
+  int kernel;                       // 0 persistent, 1 plain
+  int block_size;
+  int lds_stack;
+  int grid_blocks;                  // persistent grid
+  int count;                        // 1 = instrumented run (node / prim / neighbour counters)
+};
+hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int* blocks_per_cu);
+int default_lds_stack();
+
+// ---- post ----
+hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);
+// Converts one instance's accumulators: raw_stored[j] (local stored order)
+// and, in input order via perm, raw_out / sigma_out / cnt_out (each may be null).
+hipError_t finalize_instance(const unsigned long long* s1, const unsigned long long* s2,
+                             const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+                             double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
+                             double* sigma_out, unsigned long long* cnt_out, hipStream_t stream);
+// Signed fixed-point (adjoint) accumulator to double, input order.
+hipError_t finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+                           double inv_scale, double* out, hipStream_t stream);
+// Smoothed, max-normalised estimate (reference device.hpp:153-187), written
+// in input order. raw_stored / areas are in local stored order; the CSR uses
+// absolute ids. vtmp: n doubles; maxbuf: 1 double.
+hipError_t smooth_normalize(const double* raw_stored, const float* areas_stored, const uint32_t* perm,
+                            const uint32_t* nb_ptr, const uint32_t* nb_col, uint32_t prim_begin, uint32_t n,
+                            double* vtmp, double* maxbuf, float* out, hipStream_t stream);
+// Permute a stored-order float array into input order.
+hipError_t permute_f32(const float* stored, const uint32_t* perm, uint32_t n, float* out, hipStream_t stream);
+
+}  // namespace rti_impl

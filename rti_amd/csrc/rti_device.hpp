@@ -1,0 +1,83 @@
+// Device-side data layout and helpers shared by the gfx950 kernels.
+//
+// Layout in HBM (one "scene store" holds one geometry or a whole batch; all
+// arrays are concatenated over the instances and indexed by absolute ids):
+//   prim_a[p]  float4   discs: centre xyz + radius;   triangles: v0, w = -1
+//   prim_b[p]  float4   discs: unit normal xyz, 0;    triangles: e1 = v1 - v0
+//   prim_c[p]  float4   triangles: e2 = v2 - v0 (unused for discs)
+//   nodes[i]   bvh_node 64 B, both children's boxes in the parent
+//   nb_ptr/nb_col       neighbour CSR over absolute prim ids (discs)
+//   acc_*[p]            64-bit accumulators
+// Primitives are stored in Morton order of their instance, so neighbouring
+// primitives in space are neighbours in memory (coalesced leaf and
+// accumulator traffic); `perm` maps a stored position back to the input id.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rti_dev {
+
+// Internal node of the binary LBVH. Child c's box is
+//   x: (lo_hi_xy[c].x, lo_hi_xy[c].y), y: (lo_hi_xy[c].z, lo_hi_xy[c].w),
+//   z: (lo_hi_z.{x,y} for child 0, lo_hi_z.{z,w} for child 1).
+// child[c] >= 0 is an internal node index; child[c] < 0 is leaf ~prim.
+struct __align__(16) bvh_node {
+  float4 xy0;     // child 0: lo.x, hi.x, lo.y, hi.y
+  float4 xy1;     // child 1: lo.x, hi.x, lo.y, hi.y
+  float4 z01;     // child 0 lo.z, hi.z, child 1 lo.z, hi.z
+  int4 child;     // child0, child1, unused, unused
+};
+static_assert(sizeof(bvh_node) == 64, "node must be one 64-byte record");
+
+// Per-instance description, read by each lane for the instance its ray
+// belongs to. 128 bytes.
+struct __align__(16) inst_desc {
+  float xmin, xmax, ymin, ymax;     // boundary faces
+  float zmin, zmax, src_z, tnear;   // boundary z extent, source plane, ray epsilon
+  float sticking, cos_exponent, diffuse_prob, pow_e;  // pow_e = 2 / (n + 1)
+  int32_t bc_x, bc_y, prim_kind, root;                // root: absolute node index
+  uint64_t ray_begin, n_rays;       // slice of the global ray queue
+  uint32_t seed, key1, roulette, prim_begin;
+  uint32_t n_prims, pad0, pad1, pad2;
+  float score_c, fx_scale, pad4, pad5;  // score_c = 1 / (n + 1); fixed-point scale 2^shift
+};
+static_assert(sizeof(inst_desc) == 128, "descriptor size");
+
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 make(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 normalize(f3 a) {
+  float l = sqrtf(dot(a, a));
+  return {a.x / l, a.y / l, a.z / l};
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds of the published
+// multipliers, Weyl key schedule.
+struct u4 {
+  uint32_t x, y, z, w;
+};
+__device__ __forceinline__ u4 philox(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = {hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+// 24-bit uniforms, exactly representable in float.
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
+__device__ __forceinline__ float u01_open0(uint32_t x) { return (float)((x >> 8) + 1u) * 5.9604644775390625e-08f; }
+
+}  // namespace rti_dev

@@ -1,0 +1,493 @@
+// Monte Carlo flux trace for gfx950.
+//
+// Replaces the per-ray loop of trace::tracer::run (reference
+// src/rti/trace/tracer.hpp:124-283): source sample, closest hit, boundary
+// (periodic / reflective), back-face pass-through, stick + deposit on the hit
+// disc and on the overlapping neighbours the ray crosses, Russian roulette,
+// diffuse / specular reflection.
+//
+// Two launch shapes share every device function:
+//   * persistent: a grid sized to the resident capacity; each wave keeps 64
+//     live rays and, after every segment, refills idle lanes from a global
+//     ray queue with one atomic per wave (ballot + popcount give each idle
+//     lane its id), so lanes never wait for the longest path of a batch;
+//   * plain: one thread per ray, traced to termination (the baseline).
+// Traversal: binary LBVH, both child boxes in the parent (one 64-byte node
+// load per visit), near child first, far child pushed on a per-lane stack
+// whose top LS entries live in LDS and the rest (at most 64 - LS, the tree
+// depth bound) in private memory.
+// Accumulation: per-primitive 64-bit integer atomics of round(v * 2^shift),
+// so totals are independent of the order of deposits (bit-exact run to run).
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "internal.hpp"
+
+using namespace rti_dev;
+
+namespace rti_impl {
+namespace {
+
+constexpr uint32_t kMaxSegmentsPerRay = 1u << 20;
+constexpr int kMaxBlock = 256;
+
+struct tparams {
+  const inst_desc* desc;
+  uint32_t n_inst;
+  uint64_t ray_lo, ray_hi;  // global ray ids traced by this launch
+  const float4* __restrict__ pa;
+  const float4* __restrict__ pb;
+  const float4* __restrict__ pc;
+  const bvh_node* __restrict__ nodes;
+  const uint32_t* __restrict__ nb_ptr;
+  const uint32_t* __restrict__ nb_col;
+  unsigned long long* s1;
+  unsigned long long* s2;
+  unsigned long long* cnt;
+  unsigned long long* ds;
+  unsigned long long* dn;
+  unsigned long long* queue;
+  unsigned long long* stats;
+};
+
+struct counters {
+  uint32_t seg = 0, hit = 0, miss = 0, trunc = 0;
+  uint32_t nodes = 0, prims = 0, nbt = 0, deps = 0;
+};
+
+// Closest primitive hit in [tnear, tbest]; ties go to the lower primitive id.
+template <int LS, bool COUNT>
+__device__ __forceinline__ bool closest_hit(const tparams& P, int root, f3 o, f3 d, float tnear, float& tbest,
+                                            uint32_t& pbest, uint32_t* lds, uint32_t* ovf, counters& C) {
+  const float tiny = 1e-20f;
+  f3 inv;
+  inv.x = 1.0f / (fabsf(d.x) > tiny ? d.x : copysignf(tiny, d.x));
+  inv.y = 1.0f / (fabsf(d.y) > tiny ? d.y : copysignf(tiny, d.y));
+  inv.z = 1.0f / (fabsf(d.z) > tiny ? d.z : copysignf(tiny, d.z));
+  const f3 oi = make(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  const int stride = blockDim.x;
+  bool found = false;
+  int sp = 0;
+  int cur = root;
+  for (;;) {
+    const bvh_node* nd = P.nodes + cur;
+    const float4 a = nd->xy0, b = nd->xy1, z = nd->z01;
+    const int4 ch = nd->child;
+    if (COUNT) C.nodes++;
+    float ax0 = fmaf(a.x, inv.x, -oi.x), ax1 = fmaf(a.y, inv.x, -oi.x);
+    float ay0 = fmaf(a.z, inv.y, -oi.y), ay1 = fmaf(a.w, inv.y, -oi.y);
+    float az0 = fmaf(z.x, inv.z, -oi.z), az1 = fmaf(z.y, inv.z, -oi.z);
+    float bx0 = fmaf(b.x, inv.x, -oi.x), bx1 = fmaf(b.y, inv.x, -oi.x);
+    float by0 = fmaf(b.z, inv.y, -oi.y), by1 = fmaf(b.w, inv.y, -oi.y);
+    float bz0 = fmaf(z.z, inv.z, -oi.z), bz1 = fmaf(z.w, inv.z, -oi.z);
+    float tmin0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tnear));
+    float tmax0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), tbest));
+    float tmin1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tnear));
+    float tmax1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), tbest));
+    bool h0 = tmin0 <= tmax0;
+    bool h1 = (tmin1 <= tmax1) && (ch.y != ch.x);
+    // leaf children are tested right away
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      bool h = c == 0 ? h0 : h1;
+      int cc = c == 0 ? ch.x : ch.y;
+      if (!(h && cc < 0)) continue;
+      if (c == 0) h0 = false; else h1 = false;
+      uint32_t p = (uint32_t)(~cc);
+      if (COUNT) C.prims++;
+      const float4 A = P.pa[p];
+      const float4 B = P.pb[p];
+      float th;
+      bool ok;
+      if (A.w > 0.0f) {
+        // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
+        // [tnear, tfar], no back-face culling, |hit - centre| < radius)
+        const f3 n = make(B.x, B.y, B.z);
+        const float dn = dot(d, n);
+        const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
+        th = dot(co, n) / dn;
+        const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
+        ok = (dn != 0.0f) && th >= tnear && th <= tbest && dot(hv, hv) < A.w * A.w;
+      } else {
+        // triangle: Moller-Trumbore
+        const float4 C2 = P.pc[p];
+        const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
+        const f3 pv = cross(d, e2);
+        const float det = dot(e1, pv);
+        const float idet = 1.0f / det;
+        const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
+        const float u = dot(tv, pv) * idet;
+        const f3 qv = cross(tv, e1);
+        const float v = dot(d, qv) * idet;
+        th = dot(e2, qv) * idet;
+        ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= tnear && th <= tbest;
+      }
+      if (ok && (th < tbest || !found || p < pbest)) {
+        tbest = th;
+        pbest = p;
+        found = true;
+      }
+    }
+    if (h0 && h1) {
+      const bool near0 = tmin0 <= tmin1;
+      const int far = near0 ? ch.y : ch.x;
+      cur = near0 ? ch.x : ch.y;
+      if (sp < LS) lds[sp * stride] = (uint32_t)far;
+      else ovf[sp - LS] = (uint32_t)far;
+      ++sp;
+    } else if (h0) {
+      cur = ch.x;
+    } else if (h1) {
+      cur = ch.y;
+    } else {
+      if (sp == 0) break;
+      --sp;
+      cur = (int)(sp < LS ? lds[sp * stride] : ovf[sp - LS]);
+    }
+  }
+  return found;
+}
+
+// Lateral boundary faces x = xmin/xmax, y = ymin/ymax over z in [zmin, zmax]
+// (src/rti/geo/boundary_x_y.hpp:147-235; no top or bottom face).
+// Face ids: 0 xmax, 1 ymin, 2 xmin, 3 ymax; -1 none.
+__device__ __forceinline__ int boundary_hit(const inst_desc& D, f3 o, f3 d, float tnear, float& tb) {
+  int face = -1;
+  float best = FLT_MAX;
+  if (d.x != 0.0f) {
+    const float tx = ((d.x > 0.0f ? D.xmax : D.xmin) - o.x) / d.x;
+    const float y = fmaf(d.y, tx, o.y), z = fmaf(d.z, tx, o.z);
+    if (tx >= tnear && y >= D.ymin && y <= D.ymax && z >= D.zmin && z <= D.zmax) {
+      best = tx;
+      face = d.x > 0.0f ? 0 : 2;
+    }
+  }
+  if (d.y != 0.0f) {
+    const float ty = ((d.y > 0.0f ? D.ymax : D.ymin) - o.y) / d.y;
+    const float x = fmaf(d.x, ty, o.x), z = fmaf(d.z, ty, o.z);
+    if (ty >= tnear && ty <= best && x >= D.xmin && x <= D.xmax && z >= D.zmin && z <= D.zmax) {
+      best = ty;
+      face = d.y > 0.0f ? 3 : 1;
+    }
+  }
+  tb = best;
+  return face;
+}
+
+// Neighbour test: restates local_intersector::intersect
+// (src/rti/trace/local_intersector.hpp:41-97).
+__device__ __forceinline__ bool local_intersect(f3 o, f3 d, float4 A, float4 B) {
+  const float pod = B.x * d.x + B.y * d.y + B.z * d.z;
+  if (pod > 0.0f) return false;
+  if (fabsf(pod) < 1e-9f) return false;
+  const float ddneg = A.x * B.x + A.y * B.y + A.z * B.z;
+  const float tt = (-(B.x * o.x + B.y * o.y + B.z * o.z) + ddneg) / pod;
+  if (tt <= 0.0f) return false;
+  const f3 e = make(o.x + d.x * tt - A.x, o.y + d.y * tt - A.y, o.z + d.z * tt - A.z);
+  return A.w > sqrtf(dot(e, e));
+}
+
+// Orthonormal basis (Duff et al., JCGT 6(1), 2017) and the cosine lobe of
+// cos_hemi::get (src/rti/ray/cos_hemi.hpp:10-45).
+__device__ __forceinline__ f3 diffuse_dir(f3 n, float r1, float r2) {
+  const float sign = copysignf(1.0f, n.z);
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const f3 b1 = make(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  const f3 b2 = make(b, sign + n.y * n.y * a, -n.y);
+  float s, c;
+  sincospif(2.0f * r1, &s, &c);
+  const float c1 = sqrtf(r2);
+  const float st = sqrtf(1.0f - r2);
+  return normalize(n * c1 + b1 * (c * st) + b2 * (s * st));
+}
+
+template <bool ADJ, bool FP64ACC, bool COUNT>
+__device__ __forceinline__ void deposit(const tparams& P, uint32_t p, float drop, float scale, float dds, float ddn,
+                                        counters& C) {
+  if (COUNT) C.deps++;
+  const double v = (double)drop;
+  if (FP64ACC) {
+    unsafeAtomicAdd(reinterpret_cast<double*>(P.s1) + p, v);
+    unsafeAtomicAdd(reinterpret_cast<double*>(P.s2) + p, v * v);
+  } else {
+    atomicAdd(P.s1 + p, (unsigned long long)__double2ll_rn(v * (double)scale));
+    atomicAdd(P.s2 + p, (unsigned long long)__double2ll_rn(v * v * (double)scale));
+  }
+  atomicAdd(P.cnt + p, 1ull);
+  if (ADJ) {
+    // derivative weights can exceed 1 (by about k s / (1 - s)); they use a
+    // scale 2^8 smaller than the primal sums
+    const double sa = (double)scale * (1.0 / 256.0);
+    atomicAdd(P.ds + p, (unsigned long long)__double2ll_rn((double)dds * sa));
+    atomicAdd(P.dn + p, (unsigned long long)__double2ll_rn((double)ddn * sa));
+  }
+}
+
+struct ray_state {
+  f3 o, d;
+  float w;
+  float pk;       // (1 - s)^k, adjoint only
+  float rfac;     // product of roulette boosts, adjoint only
+  float score;    // d log p(direction) / dn, adjoint only
+  uint64_t id;    // ray id local to the instance
+  uint32_t inst;
+  uint32_t event;
+  uint32_t k;     // front-face hits so far
+  uint32_t segs;
+};
+
+__device__ __forceinline__ uint32_t find_instance(const tparams& P, uint64_t gid) {
+  uint32_t lo = 0, hi = P.n_inst - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (P.desc[mid].ray_begin <= gid) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Primary ray: rectangle_origin_z::get (src/rti/ray/rectangle_origin_z.hpp:23-37)
+// and power_cosine_direction_z::get (src/rti/ray/power_cosine_direction_z.hpp:17-32),
+// cosine_direction_z for n = 1 (src/rti/ray/cosine_direction_z.hpp:13-27).
+__device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_state& R) {
+  R.inst = find_instance(P, gid);
+  const inst_desc& D = P.desc[R.inst];
+  R.id = gid - D.ray_begin;
+  const u4 u = philox({(uint32_t)R.id, (uint32_t)(R.id >> 32), 0u, 0u}, D.seed, D.key1);
+  R.o = make(fmaf(D.xmax - D.xmin, u01(u.x), D.xmin), fmaf(D.ymax - D.ymin, u01(u.y), D.ymin), D.src_z);
+  const float r1 = u01(u.z), r2 = u01_open0(u.w);
+  const float tt = D.cos_exponent == 1.0f ? r2 : powf(r2, D.pow_e);
+  float s, c;
+  sincospif(2.0f * r1, &s, &c);
+  const float st = sqrtf(1.0f - tt);
+  R.d = normalize(make(c * st, s * st, -sqrtf(tt)));
+  R.w = 1.0f;
+  R.pk = 1.0f;
+  R.rfac = 1.0f;
+  R.score = D.score_c + logf(r2) * D.score_c;
+  R.event = 1;
+  R.k = 0;
+  R.segs = 0;
+}
+
+// One segment of the ray (one closest-hit query and its consequences).
+// Returns false when the ray terminates.
+template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
+__device__ __forceinline__ bool step(const tparams& P, ray_state& R, uint32_t* lds, uint32_t* ovf, counters& C) {
+  const inst_desc& D = P.desc[R.inst];
+  const float tnear = D.tnear;
+  C.seg++;
+  R.segs++;
+  float tb;
+  const int face = boundary_hit(D, R.o, R.d, tnear, tb);
+  float tp = tb;
+  uint32_t prim = 0;
+  const bool hitp = closest_hit<LS, COUNT>(P, D.root, R.o, R.d, tnear, tp, prim, lds, ovf, C);
+  if (!hitp) {
+    if (face < 0) {  // tracer.hpp:184-190
+      C.miss++;
+      return false;
+    }
+    // boundary_x_y::process_hit (src/rti/geo/boundary_x_y.hpp:91-143)
+    const f3 h = R.o + R.d * tb;
+    if (face == 0 || face == 2) {
+      if (D.bc_x == 0) { R.d.x = -R.d.x; R.o = h; }
+      else R.o = make(face == 0 ? D.xmin : D.xmax, h.y, h.z);
+    } else {
+      if (D.bc_y == 0) { R.d.y = -R.d.y; R.o = h; }
+      else R.o = make(h.x, face == 1 ? D.ymax : D.ymin, h.z);
+    }
+    return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+  }
+  const float4 B = P.pb[prim];
+  f3 n;
+  const bool disc = D.prim_kind == 0;
+  if (disc) {
+    n = make(B.x, B.y, B.z);
+  } else {
+    const float4 C2 = P.pc[prim];
+    n = normalize(cross(make(B.x, B.y, B.z), make(C2.x, C2.y, C2.z)));
+  }
+  if (dot(R.d, n) > 0.0f) {  // back face: pass through (tracer.hpp:212-227)
+    R.o = R.o + R.d * tp;
+    return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+  }
+  C.hit++;
+  // stick and deposit (tracer.hpp:228-237)
+  const float s = D.sticking;
+  const float drop = R.w * s;
+  float dds = 0.0f, ddn = 0.0f;
+  if (ADJ) {
+    // d/ds of s (1-s)^k, times the roulette boosts of this path
+    const float g = (R.k == 0) ? 1.0f : R.pk * (1.0f - (float)R.k * s / (1.0f - s));
+    dds = g * R.rfac;
+    ddn = drop * R.score;
+  }
+  const float scale = D.fx_scale;
+  deposit<ADJ, FP64ACC, COUNT>(P, prim, drop, scale, dds, ddn, C);
+  if (disc) {  // check_for_additional_intersections (tracer.hpp:339-383)
+    const uint32_t q0 = P.nb_ptr[prim], q1 = P.nb_ptr[prim + 1];
+    for (uint32_t q = q0; q < q1; ++q) {
+      const uint32_t j = P.nb_col[q];
+      if (COUNT) C.nbt++;
+      if (local_intersect(R.o, R.d, P.pa[j], P.pb[j])) deposit<ADJ, FP64ACC, COUNT>(P, j, drop, scale, dds, ddn, C);
+    }
+  }
+  R.w -= drop;
+  if (R.w == 0.0f) return false;  // tracer.hpp:238-240
+  R.k++;
+  if (ADJ) R.pk *= (1.0f - s);
+  const u4 u = philox({(uint32_t)R.id, (uint32_t)(R.id >> 32), R.event, 0u}, D.seed, D.key1);
+  R.event++;
+  // rejection_control::check_weight_reweight_or_kill (src/rti/mc/rejection_control.hpp:27-61)
+  if (D.roulette && R.w < 0.1f) {
+    if (u01(u.x) < 1.0f - R.w / 0.3f) return false;
+    if (ADJ) R.rfac *= 0.3f / R.w;
+    R.w = 0.3f;
+  }
+  R.o = R.o + R.d * tp;  // meta_geometry::get_new_origin (src/rti/geo/meta_geometry.hpp:17-23)
+  if (u01(u.w) < D.diffuse_prob) R.d = diffuse_dir(n, u01(u.y), u01(u.z));
+  else R.d = R.d - n * (2.0f * dot(R.d, n));  // specular::use (src/rti/reflection/specular.hpp:16-40)
+  return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+}
+
+__device__ __forceinline__ void flush_counters(const tparams& P, const counters& C, bool count) {
+  // wave reduction, then one atomic per wave
+  uint32_t v[8] = {C.seg, C.hit, C.miss, C.trunc, C.nodes, C.prims, C.nbt, C.deps};
+  const int nv = count ? 8 : 4;
+  for (int i = 0; i < nv; ++i) {
+    unsigned long long x = v[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(P.stats + i, x);
+  }
+}
+
+template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
+__global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
+  extern __shared__ uint32_t smem[];
+  uint32_t* lds = smem + threadIdx.x;
+  uint32_t ovf[64 - LS];
+  counters C;
+  ray_state R;
+  bool alive = false, exhausted = false;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (;;) {
+    const unsigned long long need = __ballot(!alive && !exhausted);
+    if (need) {
+      const int cnt = __popcll(need);
+      const int leader = __ffsll((long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(P.queue, (unsigned long long)cnt);
+      base = __shfl(base, leader, 64);
+      if (!alive && !exhausted) {
+        const unsigned long long my = base + __popcll(need & lt);
+        if (my < P.ray_hi) {
+          start_ray(P, my, R);
+          alive = true;
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(alive) == 0ull) break;
+    if (alive) alive = step<LS, ADJ, FP64ACC, COUNT>(P, R, lds, ovf, C);
+  }
+  flush_counters(P, C, COUNT);
+}
+
+template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
+__global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
+  extern __shared__ uint32_t smem[];
+  uint32_t* lds = smem + threadIdx.x;
+  uint32_t ovf[64 - LS];
+  counters C;
+  ray_state R;
+  const unsigned long long gid = P.ray_lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < P.ray_hi) {
+    start_ray(P, gid, R);
+    while (step<LS, ADJ, FP64ACC, COUNT>(P, R, lds, ovf, C)) {
+    }
+  }
+  flush_counters(P, C, COUNT);
+}
+
+using kfn = void (*)(tparams);
+
+template <int LS>
+kfn pick(bool persistent, bool adj, bool fp64, bool count) {
+  if (count) return persistent ? k_trace_persistent<LS, false, false, true> : k_trace_plain<LS, false, false, true>;
+  if (persistent) {
+    if (adj) return fp64 ? k_trace_persistent<LS, true, true, false> : k_trace_persistent<LS, true, false, false>;
+    return fp64 ? k_trace_persistent<LS, false, true, false> : k_trace_persistent<LS, false, false, false>;
+  }
+  if (adj) return fp64 ? k_trace_plain<LS, true, true, false> : k_trace_plain<LS, true, false, false>;
+  return fp64 ? k_trace_plain<LS, false, true, false> : k_trace_plain<LS, false, false, false>;
+}
+
+}  // namespace
+
+int default_lds_stack() { return 16; }
+
+hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
+  tparams P;
+  P.desc = L.desc;
+  P.n_inst = L.n_inst;
+  P.ray_lo = L.ray_lo;
+  P.ray_hi = L.ray_hi;
+  P.pa = L.prim_a;
+  P.pb = L.prim_b;
+  P.pc = L.prim_c;
+  P.nodes = L.nodes;
+  P.nb_ptr = L.nb_ptr;
+  P.nb_col = L.nb_col;
+  P.s1 = L.acc_s1;
+  P.s2 = L.acc_s2;
+  P.cnt = L.acc_cnt;
+  P.ds = L.acc_ds;
+  P.dn = L.acc_dn;
+  P.queue = L.queue;
+  P.stats = L.stats;
+  const bool persistent = L.kernel == 0;
+  const bool count = L.count != 0;
+  const int bs = L.block_size;
+  if (bs < 64 || bs > kMaxBlock || (bs & 63)) return hipErrorInvalidValue;
+  kfn f;
+  switch (L.lds_stack) {
+    case 4: f = pick<4>(persistent, L.adjoint, L.accum_fp64, count); break;
+    case 8: f = pick<8>(persistent, L.adjoint, L.accum_fp64, count); break;
+    case 16: f = pick<16>(persistent, L.adjoint, L.accum_fp64, count); break;
+    case 24: f = pick<24>(persistent, L.adjoint, L.accum_fp64, count); break;
+    case 32: f = pick<32>(persistent, L.adjoint, L.accum_fp64, count); break;
+    default: return hipErrorInvalidValue;
+  }
+  const size_t shmem = (size_t)bs * L.lds_stack * 4;
+  uint64_t grid;
+  if (persistent) {
+    grid = (uint64_t)L.grid_blocks;
+  } else {
+    grid = (L.ray_hi - L.ray_lo + bs - 1) / bs;
+    if (grid == 0) grid = 1;
+  }
+  hipLaunchKernelGGL(f, dim3((unsigned)grid), dim3(bs), shmem, stream, P);
+  return hipGetLastError();
+}
+
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int* blocks_per_cu) {
+  kfn f;
+  switch (lds_stack) {
+    case 4: f = pick<4>(persistent, adjoint, fp64, false); break;
+    case 8: f = pick<8>(persistent, adjoint, fp64, false); break;
+    case 16: f = pick<16>(persistent, adjoint, fp64, false); break;
+    case 24: f = pick<24>(persistent, adjoint, fp64, false); break;
+    case 32: f = pick<32>(persistent, adjoint, fp64, false); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block_size,
+                                                      (size_t)block_size * lds_stack * 4);
+}
+
+}  // namespace rti_impl
