@@ -159,6 +159,14 @@ int rti_get_adjoint(rti_ctx* ctx, double* draw_ds, double* draw_dn);
  * [3] truncated rays; after an RTI_KERNEL_COUNT run also [4] BVH node
  * visits, [5] primitive tests, [6] neighbour tests, [7] deposits. */
 int rti_get_counters(rti_ctx* ctx, uint64_t out[8]);
+/* The built hierarchy of the (first) geometry, for validation and tools:
+ * n_nodes 64-byte nodes {float4 child0 x/y box, float4 child1 x/y box,
+ * float4 z boxes, int4 children (>= 0 node, < 0 leaf ~stored position)},
+ * perm[stored position] = input id, prim_a (n_prims x float4, stored
+ * order: disc centre + radius, or triangle v0 with w = -1). Node indices in
+ * the children are relative to this geometry. Any output may be NULL. */
+int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* perm, uint64_t* n_nodes,
+                      float* prim_a);
 
 /* Batch: many geometries in one persistent launch (captured in a HIP graph
  * and replayed by rti_batch_replay). Results per instance by index. */

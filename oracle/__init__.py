@@ -22,6 +22,35 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_ref(reference="/root/reference"):
+    """Compile the reference's Embree-free headers into oracle/_ref (only
+    where the reference checkout exists)."""
+    if os.path.isdir(os.path.join(reference, "src", "rti")):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REF={reference}"], check=True)
+        return True
+    return False
+
+
+def ref_lib():
+    """The compiled reference headers (oracle/_ref/libref.so) or None."""
+    path = os.path.join(_HERE, "_ref", "libref.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.ref_area_inside.argtypes = [ctypes.c_float] * 4 + [_fp, _fp]
+    L.ref_area_inside.restype = ctypes.c_float
+    L.ref_neighbors.argtypes = [_fp, ctypes.c_uint64, _u32p, _u32p]
+    L.ref_neighbors.restype = ctypes.c_int64
+    L.ref_orthonormal_basis.argtypes = [_fp, _fp]
+    L.ref_cos_hemi.argtypes = [_fp, _u64p, _fp]
+    L.ref_power_cosine_direction.argtypes = [ctypes.c_float, _u64p, _fp]
+    L.ref_rectangle_origin.argtypes = [ctypes.c_float] * 5 + [_u64p, _fp]
+    L.ref_rejection_control.argtypes = [_fp, ctypes.c_uint64]
+    L.ref_rejection_control.restype = ctypes.c_int
+    L.ref_hit_accumulator.argtypes = [ctypes.c_uint64, _u32p, _fp, ctypes.c_uint64, _dp, _u64p, _dp, _dp]
+    return L
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -80,6 +109,10 @@ def _declare(L):
     L.orc_neighbors.restype = ctypes.c_int64
     L.orc_smooth_normalize.argtypes = [_dp, _fp, _u32p, _u32p, ctypes.c_uint64, _fp]
     L.orc_trace.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.POINTER(Result)]
+    L.orc_source_dir.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+    L.orc_diffuse_dir.argtypes = [_fp, ctypes.c_float, ctypes.c_float, _fp]
+    L.orc_roulette.argtypes = [_fp, ctypes.c_float]
+    L.orc_roulette.restype = ctypes.c_int
     L.orc_trace.restype = ctypes.c_int
 
 
@@ -117,6 +150,25 @@ def neighbors(xyzr):
     col = np.zeros(max(int(e), 1), dtype=np.uint32)
     lib().orc_neighbors(_ptr(x, _fp), n, _ptr(row, _u32p), _ptr(col, _u32p))
     return row, col[: int(e)]
+
+
+def source_dir(r1, r2, n):
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_source_dir(float(r1), float(r2), float(n), _ptr(out, _fp))
+    return out
+
+
+def diffuse_dir(nrm, r1, r2):
+    nn = np.ascontiguousarray(nrm, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_diffuse_dir(_ptr(nn, _fp), float(r1), float(r2), _ptr(out, _fp))
+    return out
+
+
+def roulette(w, u):
+    ww = np.array([w], dtype=np.float32)
+    keep = lib().orc_roulette(_ptr(ww, _fp), float(u))
+    return bool(keep), float(ww[0])
 
 
 def smooth_normalize(values, areas, row, col):

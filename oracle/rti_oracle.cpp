@@ -839,6 +839,26 @@ int trace_impl(const orc_scene* sc, const orc_params* p, orc_result* res) {
 
 extern "C" {
 
+// Samplers, exposed for the distribution and CRN tests.
+void orc_source_dir(float r1, float r2, float n, float out[3]) {
+  float lncos;
+  v3<float> d = source_dir<float>(r1, r2, n, lncos);
+  out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+void orc_diffuse_dir(const float nrm[3], float r1, float r2, float out[3]) {
+  v3<float> d = diffuse_dir<float>({nrm[0], nrm[1], nrm[2]}, r1, r2);
+  out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+// Rejection control decision for weight w and uniform u (returns 1 = continue).
+int orc_roulette(float* w, float u) {
+  if (*w >= 0.1f) return 1;
+  if (u < 1.0f - *w / 0.3f) return 0;
+  *w = 0.3f;
+  return 1;
+}
+
 int orc_trace(const orc_scene* sc, const orc_params* p, orc_result* res) {
   if (!sc || !p || !res) return 1;
   if (sc->n_prims == 0) return 2;

@@ -32,6 +32,7 @@ EXPORTS = [
     "rti_default_params", "rti_ctx_create", "rti_ctx_destroy", "rti_last_error", "rti_set_discs",
     "rti_set_triangles", "rti_set_params", "rti_run", "rti_prepare", "rti_trace", "rti_finalize", "rti_get_flux",
     "rti_get_counts", "rti_get_areas", "rti_get_neighbors", "rti_get_stats", "rti_get_adjoint", "rti_get_counters",
+    "rti_get_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
     "rti_batch_get_stats",
 ]
@@ -118,6 +119,7 @@ def _declare(L):
     L.rti_get_stats.argtypes = [vp, c.POINTER(Stats)]
     L.rti_get_adjoint.argtypes = [vp, _dp, _dp]
     L.rti_get_counters.argtypes = [vp, _u64p]
+    L.rti_get_hierarchy.argtypes = [vp, c.c_void_p, c.c_uint64, _u32p, _u64p, _fp]
     L.rti_batch_set.argtypes = [vp, c.POINTER(Instance), c.c_uint32]
     L.rti_batch_replay.argtypes = [vp, c.c_uint32]
     L.rti_batch_get_flux.argtypes = [vp, c.c_uint32, _fp, _dp, _dp]
@@ -242,6 +244,18 @@ class Context:
         self._check(lib().rti_get_neighbors(self._h, _ptr(row, _u32p), _ptr(col, _u32p), col.size,
                                             ctypes.byref(ne)))
         return row, col[: int(ne.value)]
+
+    def hierarchy(self):
+        """(nodes structured array, perm, prim_a) of the built hierarchy."""
+        nn = ctypes.c_uint64(0)
+        self._check(lib().rti_get_hierarchy(self._h, None, 0, None, ctypes.byref(nn), None))
+        dt = np.dtype([("xy0", "<f4", 4), ("xy1", "<f4", 4), ("z01", "<f4", 4), ("child", "<i4", 4)])
+        nodes = np.zeros(int(nn.value), dtype=dt)
+        perm = np.zeros(self.n, dtype=np.uint32)
+        pa = np.zeros((self.n, 4), dtype=np.float32)
+        self._check(lib().rti_get_hierarchy(self._h, nodes.ctypes.data, nodes.size, _ptr(perm, _u32p),
+                                            ctypes.byref(nn), _ptr(pa, _fp)))
+        return nodes, perm, pa
 
     def stats(self):
         s = Stats()

@@ -873,6 +873,27 @@ int rti_get_counters(rti_ctx* ctx, uint64_t out[8]) {
   return RTI_OK;
 }
 
+int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* perm, uint64_t* n_nodes,
+                      float* prim_a) {
+  if (!ctx || !n_nodes) return fail(ctx, RTI_ERR_INVALID_ARG, "null argument");
+  if (ctx->inst.empty()) return fail(ctx, RTI_ERR_NO_GEOMETRY, "no geometry");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = prepare(ctx);
+  if (rc) return rc;
+  const instance& I = ctx->inst[0];
+  const uint64_t nn = std::max<uint64_t>(I.n - 1, 1);
+  *n_nodes = nn;
+  if (nodes) {
+    if (cap_nodes < nn) return fail(ctx, RTI_ERR_INVALID_ARG, "node capacity too small");
+    HIP_TRY(ctx, hipMemcpy(nodes, ctx->nodes.as<bvh_node>() + I.node_begin, nn * sizeof(bvh_node),
+                           hipMemcpyDeviceToHost));
+  }
+  if (perm) HIP_TRY(ctx, hipMemcpy(perm, ctx->perm.as<uint32_t>() + I.prim_begin, I.n * 4, hipMemcpyDeviceToHost));
+  if (prim_a)
+    HIP_TRY(ctx, hipMemcpy(prim_a, ctx->prim_a.as<float4>() + I.prim_begin, I.n * 16, hipMemcpyDeviceToHost));
+  return RTI_OK;
+}
+
 int rti_batch_set(rti_ctx* ctx, const rti_instance* in, uint32_t n) {
   if (!ctx || !in || n == 0) return fail(ctx, RTI_ERR_INVALID_ARG, "empty batch");
   std::vector<instance> v(n);

@@ -367,8 +367,22 @@ __device__ __forceinline__ bool overlap_pred(float4 p1, float4 p2) {
   if (fabsf(p1.y - p2.y) >= sr) return false;
   if (fabsf(p1.z - p2.z) >= sr) return false;
   float d1 = p1.x - p2.x, d2 = p1.y - p2.y, d3 = p1.z - p2.z;
-  float dist = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(d1, d1), __fmul_rn(d2, d2)), __fmul_rn(d3, d3)));
-  return dist < p1.w + p2.w;
+  float s = __fadd_rn(__fadd_rn(__fmul_rn(d1, d1), __fmul_rn(d2, d2)), __fmul_rn(d3, d3));
+  // The reference compares the correctly rounded FP32 sqrt(s) with sr. On the
+  // grid inputs many pairs sit exactly at distance sr, so the decision must
+  // be bit-exact without relying on the device sqrt's rounding:
+  //   sqrt_rn(s) < sr  <=>  sqrt(s) < mid  <=>  s < mid^2,
+  // with mid the midpoint of sr and its float predecessor. mid has at most 25
+  // significant bits, so mid^2 is exact in FP64; s == mid^2 is the tie, which
+  // rounds to the neighbour with an even significand.
+  const double hi = (double)sr;
+  const double lo = (double)__int_as_float(__float_as_int(sr) - 1);
+  const double mid = 0.5 * (hi + lo);
+  const double m2 = mid * mid;
+  const double sd = (double)s;
+  if (sd < m2) return true;
+  if (sd > m2) return false;
+  return (__float_as_int(sr) & 1) != 0;
 }
 
 __device__ __forceinline__ bool box_overlap(float lox, float hix, float loy, float hiy, float loz, float hiz,

@@ -1,0 +1,284 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against what fixes it independently of the
+oracle's own formula: published known-answer vectors, the reference's own
+known-answer tests (tests/golden/*.json), the reference's headers compiled by
+oracle/Makefile `ref` (oracle/_ref, built only where the reference checkout
+exists), closed forms and brute force.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from rti_amd import geometry
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def _ref():
+    L = oracle.ref_lib()
+    if L is None:
+        oracle.build_ref()
+        L = oracle.ref_lib()
+    if L is None:
+        pytest.skip("reference checkout not available: oracle/_ref not built")
+    return L
+
+
+_fp = oracle._fp
+_u32p = oracle._u32p
+_u64p = oracle._u64p
+
+
+def test_philox_random123_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    assert [hex(x) for x in oracle.philox([0, 0, 0, 0], [0, 0])] == \
+        ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(x) for x in oracle.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == \
+        ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in oracle.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                          [0xA4093822, 0x299F31D0])] == \
+        ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_local_intersector_known_answers():
+    k = json.load(open(os.path.join(GOLD, "local_intersector_kats.json")))
+    assert len(k["cases"]) == 9
+    for c in k["cases"]:
+        assert oracle.local_intersect(c["org"], c["dir"], c["disc"], c["normal"]) == c["hit"], c["name"]
+
+
+def test_local_intersector_closed_form():
+    """Ray-plane hit inside the radius, front face only, t > 0."""
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        c = rng.uniform(-1, 1, 3)
+        r = rng.uniform(0.1, 1.0)
+        o = rng.uniform(-3, 3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        dn = d @ n
+        t = (c - o) @ n / dn if dn != 0 else -1
+        h = o + t * d
+        expect = dn < -1e-6 and t > 1e-6 and np.linalg.norm(h - c) < r * (1 - 1e-5)
+        clear_miss = dn > 1e-6 or t < -1e-6 or np.linalg.norm(h - c) > r * (1 + 1e-5)
+        got = oracle.local_intersect(o, d, list(c) + [r], n)
+        if expect:
+            assert got
+        elif clear_miss:
+            assert not got
+
+
+def test_area_inside_known_answers():
+    k = json.load(open(os.path.join(GOLD, "area_inside_kats.json")))
+    assert len(k["cases"]) == 37
+    for c in k["cases"]:
+        v = oracle.area_inside(c["box"], c["disc"], c["normal"])
+        if c["tol"] == 0:
+            assert v == c["expected"], (c["name"], v, c["expected"])
+        else:
+            assert abs(v - c["expected"]) < c["tol"], (c["name"], v, c["expected"])
+
+
+def test_area_inside_closed_forms():
+    # a disc covering the whole box: area = box area / |n_z| (projection)
+    for n in ([0, 0, 1], [0.3, 0.2, 0.9], [-0.5, 0.7, 0.4]):
+        nn = np.array(n, dtype=np.float64)
+        nn /= np.linalg.norm(nn)
+        v = oracle.area_inside((-0.5, -0.25, 0.5, 0.25), [0.01, -0.02, 0.3, 40.0], nn)
+        assert abs(v - 0.5 / abs(nn[2])) < 2e-3 * (0.5 / abs(nn[2])), (n, v)
+    # circular segment: flat disc crossing one face at distance a from the centre
+    for a in (0.1, 0.5, -0.3):
+        v = oracle.area_inside((-10, -10, a, 10), [0, 0, 0, 1], [0, 0, 1])
+        ang = 2 * math.acos(a)
+        assert abs(v - (math.pi - 0.5 * (ang - math.sin(ang)))) < 1e-5
+
+
+def test_area_inside_matches_compiled_reference():
+    L = _ref()
+    rng = np.random.default_rng(1)
+    box = (-1.0, -2.0, 3.0, 1.5)
+    bad = 0
+    for _ in range(5000):
+        r = np.float32(rng.choice([0.05, 0.3, 1.0, 3.0]))
+        c = np.array([rng.uniform(-1.5, 3.5), rng.uniform(-2.5, 2), rng.uniform(-1, 1), r], np.float32)
+        n = rng.normal(size=3).astype(np.float32)
+        n /= np.linalg.norm(n)
+        a = oracle.area_inside(box, c, n)
+        b = L.ref_area_inside(*[oracle.ctypes.c_float(v) for v in box], c.ctypes.data_as(_fp), n.ctypes.data_as(_fp))
+        if not (a == b or (math.isnan(a) and math.isnan(b))):
+            bad += 1
+    assert bad == 0
+
+
+def _ref_neighbors(L, x):
+    n = x.shape[0]
+    row = np.zeros(n + 1, dtype=np.uint32)
+    e = L.ref_neighbors(x.ctypes.data_as(_fp), n, row.ctypes.data_as(_u32p), None)
+    col = np.zeros(max(e, 1), dtype=np.uint32)
+    L.ref_neighbors(x.ctypes.data_as(_fp), n, row.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p))
+    return row, col[:e]
+
+
+@pytest.mark.parametrize("which", ["trench_c1", "hole", "rough"])
+def test_neighbors_match_compiled_reference(which):
+    """The reference's divide-and-conquer neighbourhood (run from its own
+    header) and the oracle's grid search give the same graph, including the
+    FP32-borderline diagonal pairs of the 0.02 grid (2 r = spacing * sqrt 2)."""
+    L = _ref()
+    if which == "trench_c1":
+        sc = geometry.trench_discs()
+    elif which == "hole":
+        sc = geometry.hole_discs(spacing=0.04)
+    else:
+        sc = geometry.rough_surface_discs(nfield=256, cells=1, spacing=0.02)
+    x = np.ascontiguousarray(sc["xyzr"], dtype=np.float32)
+    row, col = _ref_neighbors(L, x)
+    orow, ocol = oracle.neighbors(x)
+    assert np.array_equal(row, orow)
+    assert np.array_equal(col, ocol)
+
+
+def test_neighbors_brute_force():
+    rng = np.random.default_rng(2)
+    # a jittered grid puts many pairs near the contact distance
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), np.arange(3)), -1).reshape(-1, 3) * 0.1
+    x = np.zeros((g.shape[0], 4), np.float32)
+    x[:, :3] = g + rng.normal(scale=0.003, size=g.shape)
+    x[:, 3] = np.float32(0.1 * math.sqrt(2) / 2)
+    row, col = oracle.neighbors(x)
+    n = x.shape[0]
+    for i in range(n):
+        d = (x[:, :3] - x[i, :3]).astype(np.float32)
+        s = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(np.float32) + d[:, 2] * d[:, 2]
+        m = np.sqrt(s.astype(np.float32)) < (x[:, 3] + x[i, 3])
+        m[i] = False
+        assert np.array_equal(np.nonzero(m)[0], col[row[i]:row[i + 1]])
+
+
+def test_source_direction_matches_compiled_reference():
+    """Same uniforms in, same direction out (power_cosine_direction_z)."""
+    L = _ref()
+    rng = np.random.default_rng(3)
+    for n in (1.0, 5.0, 10.0):
+        for _ in range(500):
+            a, b = rng.integers(1, 1 << 24, 2)
+            draws = np.array([int(a) << 40, int(b) << 40], dtype=np.uint64)
+            out = np.zeros(3, np.float32)
+            L.ref_power_cosine_direction(n, draws.ctypes.data_as(_u64p), out.ctypes.data_as(_fp))
+            mine = oracle.source_dir(a * 2.0 ** -24, b * 2.0 ** -24, n)
+            np.testing.assert_allclose(mine, out, atol=2e-6)
+
+
+def test_source_direction_moments():
+    """cos theta = U^(1/(n+1)): E[cos] = (n+1)/(n+2), azimuth uniform."""
+    rng = np.random.default_rng(4)
+    for n in (1.0, 10.0):
+        u = rng.random((40000, 2))
+        d = np.array([oracle.source_dir(a, 1.0 - b, n) for a, b in u])
+        c = -d[:, 2]
+        assert abs(c.mean() - (n + 1) / (n + 2)) < 4 * c.std() / math.sqrt(len(c))
+        assert abs(d[:, 0].mean()) < 0.02 and abs(d[:, 1].mean()) < 0.02
+        np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1, atol=1e-6)
+
+
+def test_diffuse_direction_is_cosine_lobe_like_reference():
+    """Own orthonormal basis (Duff et al. 2017) vs the reference's: the same
+    cosine-weighted lobe about the normal (moments), for reference draws too."""
+    L = _ref()
+    rng = np.random.default_rng(5)
+    for nrm in ([0, 0, 1], [1, 0, 0], [0.3, -0.4, -0.866], [0, 0, -1]):
+        n = np.array(nrm, dtype=np.float32)
+        n /= np.linalg.norm(n)
+        mine, ref = [], []
+        for _ in range(20000):
+            a, b = rng.integers(0, 1 << 24, 2)
+            mine.append(oracle.diffuse_dir(n, a * 2.0 ** -24, b * 2.0 ** -24))
+            draws = np.array([int(a) << 40, int(b) << 40], dtype=np.uint64)
+            o = np.zeros(3, np.float32)
+            L.ref_cos_hemi(n.ctypes.data_as(_fp), draws.ctypes.data_as(_u64p), o.ctypes.data_as(_fp))
+            ref.append(o)
+        for arr in (np.array(mine), np.array(ref)):
+            c = arr @ n
+            assert c.min() > -1e-6
+            assert abs(c.mean() - 2 / 3) < 0.01       # E[cos] = 2/3 for a Lambertian lobe
+            assert abs((c * c).mean() - 0.5) < 0.01   # E[cos^2] = 1/2
+            t = arr - np.outer(c, n)
+            assert np.abs(t.mean(0)).max() < 0.015    # no tangential bias
+
+
+def test_roulette_matches_compiled_reference():
+    L = _ref()
+    for w in (0.5, 0.1, 0.0999, 0.05, 0.01, 0.2999):
+        for u in np.linspace(0, 1, 97, endpoint=False):
+            draw = int(u * 2 ** 24) << 40
+            ww = np.array([w], np.float32)
+            rk = L.ref_rejection_control(ww.ctypes.data_as(_fp), draw)
+            keep, w2 = oracle.roulette(w, draw / 2.0 ** 64)
+            assert keep == bool(rk), (w, u)
+            if keep:
+                assert w2 == float(ww[0])
+
+
+def test_roulette_unbiased():
+    rng = np.random.default_rng(6)
+    for w in (0.09, 0.05, 0.01):
+        us = rng.random(200000)
+        tot = 0.0
+        for u in us[:20000]:
+            keep, w2 = oracle.roulette(w, u)
+            tot += w2 if keep else 0.0
+        assert abs(tot / 20000 - w) < 4 * math.sqrt(w * 0.3 / 20000)
+
+
+def test_smooth_normalize_closed_form():
+    v = np.array([1.0, 2.0, 4.0, 0.0])
+    a = np.array([1.0, 2.0, 1.0, 1.0], np.float32)
+    row = np.array([0, 1, 3, 4, 4], np.uint32)
+    col = np.array([1, 0, 2, 1], np.uint32)
+    out = oracle.smooth_normalize(v, a, row, col)
+    raw = np.array([(1 + 1) / 2, (1 + 1 + 4) / 3, (4 + 1) / 2, 0.0])
+    np.testing.assert_allclose(out, raw / raw.max(), rtol=1e-6)
+
+
+def test_trace_flat_plate_flux_and_counts():
+    """Direct flux on a periodic flat plate is 1 for any cos^n source; the
+    number of segments equals rays + boundary crossings."""
+    sc = geometry.flat_plate_discs(size=1.0, spacing=0.05)
+    box, sz = geometry.scene_box(sc)
+    for nexp in (1.0, 10.0):
+        r = oracle.trace(sc, n_rays=100000, box=box, src_z=sz, sticking=1.0, bc_x=1, bc_y=1, cos_exponent=nexp)
+        areas = np.array([oracle.area_inside((box[0], box[2], box[1], box[3]), sc["xyzr"][i], sc["normals"][i])
+                          for i in range(sc["xyzr"].shape[0])])
+        flux = r["s1"] / areas * (box[1] - box[0]) * (box[3] - box[2]) / 100000
+        assert abs(flux.mean() - 1) < 0.02
+        assert r["surface_hits"] + r["misses"] == 100000
+        assert r["misses"] < 100
+
+
+def test_trace_path_weight_identity():
+    """Without roulette a history that makes K front hits deposits
+    s sum_{k<K} (1-s)^k = 1 - (1-s)^K on the surface (neighbour copies
+    excluded: a scene without overlaps), and d/ds of that is K (1-s)^(K-1)."""
+    sc = geometry.hole_triangles(radius=0.5, depth=1.0, half=1.0, n_az=32, n_ax=8, n_floor=4, n_top=4)
+    box, sz = geometry.scene_box(sc)
+    s = 0.3
+    r = oracle.trace(sc, n_rays=20000, box=box, src_z=sz, sticking=s, bc_x=1, bc_y=1, roulette=False, adjoint=True)
+    # total deposit per ray summed over the surface is 1 - (1-s)^K; its mean
+    # equals 1 - E[(1-s)^K], and the total d/ds equals E[K (1-s)^(K-1)].
+    # With no roulette both are functions of the per-ray hit count; check the
+    # identity d/ds sum = sum over k of d/ds of s (1-s)^k through a second run
+    # at s + h with the same paths (paths do not depend on s).
+    h = 1e-3
+    r2 = oracle.trace(sc, n_rays=20000, box=box, src_z=sz, sticking=s + h, bc_x=1, bc_y=1, roulette=False)
+    r1 = oracle.trace(sc, n_rays=20000, box=box, src_z=sz, sticking=s - h, bc_x=1, bc_y=1, roulette=False)
+    fd = (r2["s1"].sum() - r1["s1"].sum()) / (2 * h)
+    assert abs(r["ds"].sum() - fd) < 1e-3 * abs(fd)
+    assert r["surface_hits"] > 0
