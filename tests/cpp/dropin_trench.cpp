@@ -1,0 +1,83 @@
+// A caller of the rti::device drop-in, written the way the reference's own
+// example uses its API (src/rti/main/test2.cpp:22-45): a constant-sticking
+// particle, diffuse reflection, points / normals / grid spacing setters.
+// Usage: dropin_trench <out.txt> [rays] [sticking] [cos_exponent]
+// Writes one line per disc: mc estimate, hit count.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "rti/device.hpp"
+
+using numeric_type = float;
+
+class particle_t : public rti::particle::i_particle<numeric_type> {
+ public:
+  numeric_type get_sticking_probability(RTCRay&, RTCHit&, rti::geo::meta_geometry<numeric_type>&, rti::rng::i_rng&,
+                                        rti::rng::i_rng::i_state&) override final {
+    return sticking;
+  }
+  void init_new() override final {}
+  static numeric_type sticking;
+};
+numeric_type particle_t::sticking = 0.1f;
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s out.txt [rays] [sticking] [n]\n", argv[0]);
+    return 2;
+  }
+  size_t rays = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 100000;
+  particle_t::sticking = argc > 3 ? (numeric_type)std::atof(argv[3]) : 0.1f;
+  numeric_type nexp = argc > 4 ? (numeric_type)std::atof(argv[4]) : 1.0f;
+  // trench: width 1, depth 2, flats 0.5, y in [0, 0.5], 0.05 grid (same
+  // sampling as rti_amd.geometry.trench_discs(depth=2, flat=0.5, ylen=0.5, spacing=0.05))
+  const double h = 0.05, w2 = 0.5, depth = 2.0, flat = 0.5;
+  std::vector<std::array<double, 4>> prof;  // x, z, nx, nz
+  const double len = 2 * flat + 2 * depth + 2 * w2;
+  const int npts = (int)std::lround(len / h);
+  const double seg[5] = {flat, depth, 2 * w2, depth, flat};
+  const double vx[6] = {-w2 - flat, -w2, -w2, w2, w2, w2 + flat};
+  const double vz[6] = {depth, depth, 0, 0, depth, depth};
+  const double nx[5] = {0, 1, 0, -1, 0}, nz[5] = {1, 0, 1, 0, 1};
+  for (int i = 0; i <= npts; ++i) {
+    double s = i * h, acc = 0;
+    int k = 0;
+    while (k < 4 && s > acc + seg[k] + 1e-12) acc += seg[k++];
+    double t = (s - acc) / seg[k];
+    double x = vx[k] + (vx[k + 1] - vx[k]) * t, z = vz[k] + (vz[k + 1] - vz[k]) * t;
+    double ax = nx[k], az = nz[k];
+    if (std::fabs(t) < 1e-9 && k > 0) { ax += nx[k - 1]; az += nz[k - 1]; }
+    else if (std::fabs(t - 1) < 1e-9 && k < 4) { ax += nx[k + 1]; az += nz[k + 1]; }
+    double l = std::sqrt(ax * ax + az * az);
+    prof.push_back({x, z, ax / l, az / l});
+  }
+  std::vector<std::array<numeric_type, 3>> points, normals;
+  std::vector<numeric_type> spacing;
+  const int rows = (int)std::lround(0.5 / h) + 1;
+  for (int r = 0; r < rows; ++r)
+    for (auto& p : prof) {
+      points.push_back({(numeric_type)p[0], (numeric_type)(r * h), (numeric_type)p[1]});
+      normals.push_back({(numeric_type)p[2], 0.0f, (numeric_type)p[3]});
+      spacing.push_back((numeric_type)(h * std::sqrt(2.0) / 2.0));
+    }
+  auto dev = rti::device<numeric_type, particle_t, rti::reflection::diffuse<numeric_type>>{};
+  dev.set_points(points);
+  dev.set_normals(normals);
+  dev.set_grid_spacing(spacing);
+  dev.set_number_of_rays(rays);
+  dev.set_x(rti::geo::bound_condition::REFLECTIVE);
+  dev.set_y(rti::geo::bound_condition::PERIODIC);
+  auto dir = rti::ray::power_cosine_direction_z<numeric_type>(nexp);
+  dev.set(dir);
+  dev.run();
+  auto est = dev.get_mc_estimates();
+  auto cnt = dev.get_hit_cnts();
+  FILE* f = std::fopen(argv[1], "w");
+  if (!f) return 3;
+  for (size_t i = 0; i < est.size(); ++i) std::fprintf(f, "%.9g %zu\n", (double)est[i], cnt[i]);
+  std::fclose(f);
+  std::printf("discs %zu\n", est.size());
+  return 0;
+}

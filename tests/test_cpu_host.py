@@ -1,0 +1,174 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+what include/rti/rti_c.h declares, the drop-in header compiles against it,
+the scene generators, and the FP64 radiosity reference (pinned by the
+reference's own verification data)."""
+import ctypes
+import json
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import radiosity
+from rti_amd import _lib, geometry
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "rti", "rti_c.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rti_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    missing = [f for f in declared if not hasattr(L, f)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == declared
+
+
+def test_default_params_follow_the_reference():
+    p = _lib.default_params()
+    assert p.n_rays == 1024                      # device.hpp:289
+    assert p.bc_x == _lib.BC_REFLECTIVE and p.bc_y == _lib.BC_REFLECTIVE   # device.hpp:292-293
+    assert abs(p.tnear - 1e-4) < 1e-10          # tracer.hpp:179
+    assert p.cos_exponent == 1.0 and p.roulette == 1 and math.isnan(p.src_z)
+
+
+def test_null_context_is_an_error():
+    L = _lib.lib()
+    assert L.rti_run(None) == _lib.RTI_ERR_INVALID_ARG
+    assert L.rti_set_discs(None, None, None, 0) == _lib.RTI_ERR_INVALID_ARG
+    assert L.rti_ctx_create(None, 0) == _lib.RTI_ERR_INVALID_ARG
+
+
+def test_dropin_header_compiles_and_links(tmp_path):
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_trench.cpp")
+    out = tmp_path / "dropin"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-I{ROOT}/include", src, "-o", str(out),
+                        f"-L{ROOT}/rti_amd", "-lrti_hip"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_dropin_rejects_unsupported_reflection_at_compile_time(tmp_path):
+    code = """
+#include "rti/device.hpp"
+struct p_t : rti::particle::i_particle<float> {
+  float get_sticking_probability(RTCRay&, RTCHit&, rti::geo::meta_geometry<float>&, rti::rng::i_rng&,
+                                 rti::rng::i_rng::i_state&) override { return 0.5f; }
+  void init_new() override {}
+};
+struct my_reflection : rti::reflection::i_reflection<float> {
+  rti::util::pair<rti::util::triple<float>> use(RTCRay&, RTCHit&, rti::geo::meta_geometry<float>&,
+      rti::rng::i_rng&, rti::rng::i_rng::i_state&) override { return {}; }
+};
+int main() { rti::device<float, p_t, my_reflection> d; d.run(); }
+"""
+    f = tmp_path / "bad.cpp"
+    f.write_text(code)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(f)], capture_output=True,
+                       text=True)
+    assert r.returncode != 0 and "diffuse and specular" in r.stderr
+
+
+def test_trench_generator():
+    sc = geometry.trench_discs()
+    x, n = sc["xyzr"], sc["normals"]
+    assert x.shape == (51051, 4)
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+    np.testing.assert_allclose(x[:, 3], 0.02 * math.sqrt(2) / 2, rtol=1e-6)
+    box, sz = geometry.scene_box(sc)
+    assert box[0] == -6 and box[1] == 6 and box[2] == 0 and box[3] == 1
+    assert abs(sz - (4 + 2 * x[:, 3].max())) < 1e-6
+    # floor discs face up, walls face into the trench
+    floor = (x[:, 2] == 0) & (np.abs(x[:, 0]) < 0.49)
+    assert np.all(n[floor, 2] == 1)
+    left = (np.abs(x[:, 0] + 0.5) < 1e-6) & (x[:, 2] > 0.01) & (x[:, 2] < 3.99)
+    assert np.all(n[left, 0] == 1)
+
+
+def test_hole_mesh_generator():
+    sc = geometry.hole_triangles()
+    v = sc["verts"].astype(np.float64)
+    t = sc["tris"]
+    assert t.shape == (199360, 3)
+    e1 = v[t[:, 1]] - v[t[:, 0]]
+    e2 = v[t[:, 2]] - v[t[:, 0]]
+    nrm = np.cross(e1, e2)
+    area = 0.5 * np.linalg.norm(nrm, axis=1)
+    # wall (polygonal) + floor + top annulus = 2 x 2 cell with a hole, plus the hole's side
+    n_az = 320
+    wall = n_az * 0.5 * math.sin(2 * math.pi / n_az) * 5.0 / math.sin(math.pi / n_az) * math.sin(math.pi / n_az)
+    assert abs(area.sum() - (4.0 + 2 * n_az * 0.5 * math.sin(math.pi / n_az) * 5.0)) < 1e-3
+    # normals point into the open region: up on floor and top, towards the axis on the wall
+    c = v[t].mean(1)
+    unit = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    flat = np.abs(unit[:, 2]) > 0.5
+    assert np.all(unit[flat, 2] > 0.99)
+    radial = (c[~flat, 0] * unit[~flat, 0] + c[~flat, 1] * unit[~flat, 1])
+    assert np.all(radial < 0)
+    del wall
+
+
+def test_rough_surface_generator_small():
+    sc = geometry.rough_surface_discs(nfield=128, cells=2, spacing=0.02)
+    np.testing.assert_allclose(np.linalg.norm(sc["normals"], axis=1), 1.0, atol=1e-5)
+    assert sc["xyzr"][:, 2].min() == pytest.approx(-sc["depth"], abs=1e-6)
+
+
+def test_batch_generator():
+    b = geometry.batch_instances(count=8, total_rays=8e6)
+    assert len(b) == 8
+    assert abs(sum(x["n_rays"] for x in b) - 8e6) < 10
+    for x in b:
+        assert 1e-3 <= x["sticking"] <= 1.0
+        assert 2.0 <= x["aspect"] <= 20.0
+
+
+# ---- radiosity reference --------------------------------------------------
+def _fixture():
+    return json.load(open(os.path.join(ROOT, "tests", "golden", "radiosity_ar45.json")))
+
+
+@pytest.mark.parametrize("s", ["1", "0.1", "0.01"])
+def test_radiosity_reproduces_reference_ar45_solution(s):
+    """The reference's radiosity verification profile for an AR-45 cylinder
+    (300 wall rings of 0.15) pins the solver to ~1e-7 relative."""
+    p = _fixture()["profiles"][s]
+    r = radiosity.solve_hole(0.5, 45.0, float(s), n_wall=300, n_floor=20)
+    np.testing.assert_allclose(r["z"][:-1] + 0.075, p["position"], atol=1e-9)
+    rel = np.abs(r["wall"] - np.array(p["flux"])) / np.array(p["flux"])
+    assert rel.max() < 1e-6, rel.max()
+
+
+def test_radiosity_energy_balance():
+    """Absorbed + escaped = entered, for the hole and the trench."""
+    for s in (1.0, 0.3, 0.02):
+        r = radiosity.solve_hole(0.5, 5.0, s, n_wall=80, n_floor=10)
+        G = np.concatenate([r["wall"], r["floor"]])
+        A = r["areas"]
+        F, G0 = r["F"], r["G0"]
+        Fopen = 1.0 - F.sum(1)  # what an element sees of the opening
+        absorbed = s * np.sum(G * A)
+        escaped = (1 - s) * np.sum(G * A * Fopen)
+        assert abs(absorbed + escaped - math.pi * 0.25) < 1e-9 * math.pi
+        t = radiosity.solve_trench(1.0, 4.0, s, n_wall=60, n_floor=15)
+        L = t["lengths"]
+        Fo = 1.0 - t["F"].sum(1)
+        tot = s * np.sum(t["flux"] * L) + (1 - s) * np.sum(t["flux"] * L * Fo)
+        assert abs(tot - 1.0) < 1e-9
+
+
+def test_radiosity_direct_flux_quadrature_matches_closed_form():
+    r = radiosity.solve_hole(0.5, 5.0, 0.1, n_wall=50, n_floor=10)
+    q = radiosity.hole_direct_flux(0.5, 5.0, r["z"], r["rho"], 1.0)
+    assert np.max(np.abs(q - r["G0"]) / r["G0"]) < 2e-4
+    t = radiosity.solve_trench(1.0, 4.0, 0.1, n_wall=40, n_floor=10)
+    qt = radiosity.trench_direct_flux(t["strips"], 1.0, 4.0, 1.0)
+    assert np.max(np.abs(qt - t["G0"]) / t["G0"]) < 2e-3
