@@ -51,7 +51,7 @@ class Params(ctypes.Structure):
         ("cos_exponent", ctypes.c_float), ("diffuse_prob", ctypes.c_float), ("tnear", ctypes.c_float),
         ("roulette", ctypes.c_int32), ("src_z", ctypes.c_float), ("kernel", ctypes.c_int32),
         ("adjoint", ctypes.c_int32), ("accum", ctypes.c_int32), ("block_size", ctypes.c_int32),
-        ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+        ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
     ]
 
 

@@ -148,6 +148,7 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
       p.lds_stack != 32)
     return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 8, 16, 24 or 32");
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
+  if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
   return RTI_OK;
 }
 
@@ -495,6 +496,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.lds_stack = cfg.lds;
   L.grid_blocks = cfg.grid;
   L.count = count ? 1 : 0;
+  L.compact = ctx->params.compact_lanes;
   if (kernel == RTI_KERNEL_PERSISTENT) {
     L.ray_lo = 0;
     L.ray_hi = ctx->rays_total;

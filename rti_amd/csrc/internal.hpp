@@ -94,10 +94,12 @@ struct trace_launch {
   int lds_stack;
   int grid_blocks;                  // persistent grid
   int count;                        // 1 = instrumented run (node / prim / neighbour counters)
+  int compact;                      // persistent: lanes done before a wave shades (1..64, 0 = default)
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
 hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int* blocks_per_cu);
 int default_lds_stack();
+int default_compact_lanes();
 
 // ---- post ----
 hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);

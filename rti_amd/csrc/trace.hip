@@ -50,6 +50,7 @@ struct tparams {
   unsigned long long* dn;
   unsigned long long* queue;
   unsigned long long* stats;
+  int compact;  // lanes without an open query before a wave shades and refills
 };
 
 struct counters {
@@ -57,97 +58,105 @@ struct counters {
   uint32_t nodes = 0, prims = 0, nbt = 0, deps = 0;
 };
 
+// Resumable closest-hit traversal (one node per trav_step), so that a wave
+// can stop traversing once enough of its lanes have finished their query.
 // Closest primitive hit in [tnear, tbest]; ties go to the lower primitive id.
+struct trav_state {
+  f3 inv, oi;       // 1 / direction, origin / direction
+  float tnear;
+  float tbest;      // starts at the boundary-face distance
+  float tb;         // boundary-face distance
+  int face;         // boundary face hit, -1 none
+  int cur;          // node being visited
+  int sp;           // stack depth
+  uint32_t pbest;   // closest primitive so far
+  bool found;
+};
+
 template <int LS, bool COUNT>
-__device__ __forceinline__ bool closest_hit(const tparams& P, int root, f3 o, f3 d, float tnear, float& tbest,
-                                            uint32_t& pbest, uint32_t* lds, uint32_t* ovf, counters& C) {
-  const float tiny = 1e-20f;
-  f3 inv;
-  inv.x = 1.0f / (fabsf(d.x) > tiny ? d.x : copysignf(tiny, d.x));
-  inv.y = 1.0f / (fabsf(d.y) > tiny ? d.y : copysignf(tiny, d.y));
-  inv.z = 1.0f / (fabsf(d.z) > tiny ? d.z : copysignf(tiny, d.z));
-  const f3 oi = make(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+__device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint32_t* lds,
+                                          uint32_t* ovf, counters& C) {
   const int stride = blockDim.x;
-  bool found = false;
-  int sp = 0;
-  int cur = root;
-  for (;;) {
-    const bvh_node* nd = P.nodes + cur;
-    const float4 a = nd->xy0, b = nd->xy1, z = nd->z01;
-    const int4 ch = nd->child;
-    if (COUNT) C.nodes++;
-    float ax0 = fmaf(a.x, inv.x, -oi.x), ax1 = fmaf(a.y, inv.x, -oi.x);
-    float ay0 = fmaf(a.z, inv.y, -oi.y), ay1 = fmaf(a.w, inv.y, -oi.y);
-    float az0 = fmaf(z.x, inv.z, -oi.z), az1 = fmaf(z.y, inv.z, -oi.z);
-    float bx0 = fmaf(b.x, inv.x, -oi.x), bx1 = fmaf(b.y, inv.x, -oi.x);
-    float by0 = fmaf(b.z, inv.y, -oi.y), by1 = fmaf(b.w, inv.y, -oi.y);
-    float bz0 = fmaf(z.z, inv.z, -oi.z), bz1 = fmaf(z.w, inv.z, -oi.z);
-    float tmin0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tnear));
-    float tmax0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), tbest));
-    float tmin1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tnear));
-    float tmax1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), tbest));
-    bool h0 = tmin0 <= tmax0;
-    bool h1 = (tmin1 <= tmax1) && (ch.y != ch.x);
-    // leaf children are tested right away
+  const bvh_node* nd = P.nodes + T.cur;
+  const float4 a = nd->xy0, b = nd->xy1, z = nd->z01;
+  const int4 ch = nd->child;
+  if (COUNT) C.nodes++;
+  const f3 inv = T.inv, oi = T.oi;
+  float ax0 = fmaf(a.x, inv.x, -oi.x), ax1 = fmaf(a.y, inv.x, -oi.x);
+  float ay0 = fmaf(a.z, inv.y, -oi.y), ay1 = fmaf(a.w, inv.y, -oi.y);
+  float az0 = fmaf(z.x, inv.z, -oi.z), az1 = fmaf(z.y, inv.z, -oi.z);
+  float bx0 = fmaf(b.x, inv.x, -oi.x), bx1 = fmaf(b.y, inv.x, -oi.x);
+  float by0 = fmaf(b.z, inv.y, -oi.y), by1 = fmaf(b.w, inv.y, -oi.y);
+  float bz0 = fmaf(z.z, inv.z, -oi.z), bz1 = fmaf(z.w, inv.z, -oi.z);
+  float tmin0 = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), T.tnear));
+  float tmax0 = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), T.tbest));
+  float tmin1 = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), T.tnear));
+  float tmax1 = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), T.tbest));
+  bool h0 = tmin0 <= tmax0;
+  bool h1 = (tmin1 <= tmax1) && (ch.y != ch.x);
+  // leaf children are tested right away
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      bool h = c == 0 ? h0 : h1;
-      int cc = c == 0 ? ch.x : ch.y;
-      if (!(h && cc < 0)) continue;
-      if (c == 0) h0 = false; else h1 = false;
-      uint32_t p = (uint32_t)(~cc);
-      if (COUNT) C.prims++;
-      const float4 A = P.pa[p];
-      const float4 B = P.pb[p];
-      float th;
-      bool ok;
-      if (A.w > 0.0f) {
-        // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
-        // [tnear, tfar], no back-face culling, |hit - centre| < radius)
-        const f3 n = make(B.x, B.y, B.z);
-        const float dn = dot(d, n);
-        const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
-        th = dot(co, n) / dn;
-        const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
-        ok = (dn != 0.0f) && th >= tnear && th <= tbest && dot(hv, hv) < A.w * A.w;
-      } else {
-        // triangle: Moller-Trumbore
-        const float4 C2 = P.pc[p];
-        const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
-        const f3 pv = cross(d, e2);
-        const float det = dot(e1, pv);
-        const float idet = 1.0f / det;
-        const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
-        const float u = dot(tv, pv) * idet;
-        const f3 qv = cross(tv, e1);
-        const float v = dot(d, qv) * idet;
-        th = dot(e2, qv) * idet;
-        ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= tnear && th <= tbest;
-      }
-      if (ok && (th < tbest || !found || p < pbest)) {
-        tbest = th;
-        pbest = p;
-        found = true;
-      }
-    }
-    if (h0 && h1) {
-      const bool near0 = tmin0 <= tmin1;
-      const int far = near0 ? ch.y : ch.x;
-      cur = near0 ? ch.x : ch.y;
-      if (sp < LS) lds[sp * stride] = (uint32_t)far;
-      else ovf[sp - LS] = (uint32_t)far;
-      ++sp;
-    } else if (h0) {
-      cur = ch.x;
-    } else if (h1) {
-      cur = ch.y;
+  for (int c = 0; c < 2; ++c) {
+    bool h = c == 0 ? h0 : h1;
+    int cc = c == 0 ? ch.x : ch.y;
+    if (!(h && cc < 0)) continue;
+    if (c == 0) h0 = false; else h1 = false;
+    uint32_t p = (uint32_t)(~cc);
+    if (COUNT) C.prims++;
+    const float4 A = P.pa[p];
+    const float4 B = P.pb[p];
+    float th;
+    bool ok;
+    if (A.w > 0.0f) {
+      // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
+      // [tnear, tfar], no back-face culling, |hit - centre| < radius)
+      const f3 n = make(B.x, B.y, B.z);
+      const float dn = dot(d, n);
+      const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
+      th = dot(co, n) / dn;
+      const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
+      ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
     } else {
-      if (sp == 0) break;
-      --sp;
-      cur = (int)(sp < LS ? lds[sp * stride] : ovf[sp - LS]);
+      // triangle: Moller-Trumbore
+      const float4 C2 = P.pc[p];
+      const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
+      const f3 pv = cross(d, e2);
+      const float det = dot(e1, pv);
+      const float idet = 1.0f / det;
+      const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
+      const float u = dot(tv, pv) * idet;
+      const f3 qv = cross(tv, e1);
+      const float v = dot(d, qv) * idet;
+      th = dot(e2, qv) * idet;
+      ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= T.tnear && th <= T.tbest;
+    }
+    if (ok && (th < T.tbest || !T.found || p < T.pbest)) {
+      T.tbest = th;
+      T.pbest = p;
+      T.found = true;
     }
   }
-  return found;
+  if (h0 && h1) {
+    const bool near0 = tmin0 <= tmin1;
+    const int far = near0 ? ch.y : ch.x;
+    T.cur = near0 ? ch.x : ch.y;
+    if (T.sp < LS) lds[T.sp * stride] = (uint32_t)far;
+    else ovf[T.sp - LS] = (uint32_t)far;
+    ++T.sp;
+    return false;
+  }
+  if (h0) {
+    T.cur = ch.x;
+    return false;
+  }
+  if (h1) {
+    T.cur = ch.y;
+    return false;
+  }
+  if (T.sp == 0) return true;
+  --T.sp;
+  T.cur = (int)(T.sp < LS ? lds[T.sp * stride] : ovf[T.sp - LS]);
+  return false;
 }
 
 // Lateral boundary faces x = xmin/xmax, y = ymin/ymax over z in [zmin, zmax]
@@ -272,35 +281,49 @@ __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_st
   R.segs = 0;
 }
 
-// One segment of the ray (one closest-hit query and its consequences).
-// Returns false when the ray terminates.
-template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
-__device__ __forceinline__ bool step(const tparams& P, ray_state& R, uint32_t* lds, uint32_t* ovf, counters& C) {
+// Start of a segment: the boundary-face distance bounds the closest-hit
+// query, which starts at the instance's root.
+__device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C) {
   const inst_desc& D = P.desc[R.inst];
-  const float tnear = D.tnear;
   C.seg++;
   R.segs++;
-  float tb;
-  const int face = boundary_hit(D, R.o, R.d, tnear, tb);
-  float tp = tb;
-  uint32_t prim = 0;
-  const bool hitp = closest_hit<LS, COUNT>(P, D.root, R.o, R.d, tnear, tp, prim, lds, ovf, C);
-  if (!hitp) {
-    if (face < 0) {  // tracer.hpp:184-190
+  T.tnear = D.tnear;
+  T.face = boundary_hit(D, R.o, R.d, T.tnear, T.tb);
+  T.tbest = T.tb;
+  T.found = false;
+  T.pbest = 0;
+  T.cur = D.root;
+  T.sp = 0;
+  const float tiny = 1e-20f;
+  T.inv.x = 1.0f / (fabsf(R.d.x) > tiny ? R.d.x : copysignf(tiny, R.d.x));
+  T.inv.y = 1.0f / (fabsf(R.d.y) > tiny ? R.d.y : copysignf(tiny, R.d.y));
+  T.inv.z = 1.0f / (fabsf(R.d.z) > tiny ? R.d.z : copysignf(tiny, R.d.z));
+  T.oi = make(R.o.x * T.inv.x, R.o.y * T.inv.y, R.o.z * T.inv.z);
+}
+
+// The consequences of a finished closest-hit query (tracer.hpp:184-253).
+// Returns false when the ray terminates.
+template <bool ADJ, bool FP64ACC, bool COUNT>
+__device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, const trav_state& T, counters& C) {
+  const inst_desc& D = P.desc[R.inst];
+  if (!T.found) {
+    if (T.face < 0) {  // tracer.hpp:184-190
       C.miss++;
       return false;
     }
     // boundary_x_y::process_hit (src/rti/geo/boundary_x_y.hpp:91-143)
-    const f3 h = R.o + R.d * tb;
-    if (face == 0 || face == 2) {
+    const f3 h = R.o + R.d * T.tb;
+    if (T.face == 0 || T.face == 2) {
       if (D.bc_x == 0) { R.d.x = -R.d.x; R.o = h; }
-      else R.o = make(face == 0 ? D.xmin : D.xmax, h.y, h.z);
+      else R.o = make(T.face == 0 ? D.xmin : D.xmax, h.y, h.z);
     } else {
       if (D.bc_y == 0) { R.d.y = -R.d.y; R.o = h; }
-      else R.o = make(h.x, face == 1 ? D.ymax : D.ymin, h.z);
+      else R.o = make(h.x, T.face == 1 ? D.ymax : D.ymin, h.z);
     }
     return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
   }
+  const uint32_t prim = T.pbest;
+  const float tp = T.tbest;
   const float4 B = P.pb[prim];
   f3 n;
   const bool disc = D.prim_kind == 0;
@@ -365,6 +388,13 @@ __device__ __forceinline__ void flush_counters(const tparams& P, const counters&
   }
 }
 
+// Persistent, wavefront-scheduled trace. Each wave holds up to 64 live rays
+// and alternates two phases:
+//   traverse: every lane with an open closest-hit query advances one node per
+//     iteration, until at least `compact` lanes have no open query;
+//   shade + refill: the lanes whose query finished process the hit and start
+//     their next segment; lanes whose ray ended take new rays from the global
+//     queue, one atomic per wave (ballot + popcount assign consecutive ids).
 template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
   extern __shared__ uint32_t smem[];
@@ -372,9 +402,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
   uint32_t ovf[64 - LS];
   counters C;
   ray_state R;
-  bool alive = false, exhausted = false;
+  trav_state T;
+  bool alive = false, exhausted = false, tracing = false;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int compact = P.compact;
   for (;;) {
     const unsigned long long need = __ballot(!alive && !exhausted);
     if (need) {
@@ -387,18 +419,31 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
         const unsigned long long my = base + __popcll(need & lt);
         if (my < P.ray_hi) {
           start_ray(P, my, R);
+          begin_segment(P, R, T, C);
           alive = true;
+          tracing = true;
         } else {
           exhausted = true;
         }
       }
     }
     if (__ballot(alive) == 0ull) break;
-    if (alive) alive = step<LS, ADJ, FP64ACC, COUNT>(P, R, lds, ovf, C);
+    for (;;) {
+      if (tracing) tracing = !trav_step<LS, COUNT>(P, R.o, R.d, T, lds, ovf, C);
+      if (__popcll(__ballot(!tracing)) >= compact) break;
+    }
+    if (alive && !tracing) {
+      alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
+      if (alive) {
+        begin_segment(P, R, T, C);
+        tracing = true;
+      }
+    }
   }
   flush_counters(P, C, COUNT);
 }
 
+// Plain trace: one thread per ray, every segment traversed to the end.
 template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   extern __shared__ uint32_t smem[];
@@ -406,10 +451,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   uint32_t ovf[64 - LS];
   counters C;
   ray_state R;
+  trav_state T;
   const unsigned long long gid = P.ray_lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid < P.ray_hi) {
     start_ray(P, gid, R);
-    while (step<LS, ADJ, FP64ACC, COUNT>(P, R, lds, ovf, C)) {
+    for (;;) {
+      begin_segment(P, R, T, C);
+      while (!trav_step<LS, COUNT>(P, R.o, R.d, T, lds, ovf, C)) {
+      }
+      if (!finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C)) break;
     }
   }
   flush_counters(P, C, COUNT);
@@ -431,6 +481,7 @@ kfn pick(bool persistent, bool adj, bool fp64, bool count) {
 }  // namespace
 
 int default_lds_stack() { return 16; }
+int default_compact_lanes() { return 56; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   tparams P;
@@ -451,6 +502,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.dn = L.acc_dn;
   P.queue = L.queue;
   P.stats = L.stats;
+  P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();
   const bool persistent = L.kernel == 0;
   const bool count = L.count != 0;
   const int bs = L.block_size;

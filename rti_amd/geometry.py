@@ -83,6 +83,31 @@ def trench_discs(width=1.0, depth=4.0, flat=5.5, ylen=1.0, spacing=0.02, radius=
                          profile_x=px, profile_z=pz)
 
 
+def trench_triangles(width=1.0, depth=4.0, flat=5.5, ylen=1.0, spacing=0.02):
+    """The trench of ``trench_discs`` as a triangle mesh: the same profile
+    points, extruded along y in rows of ``spacing``, two triangles per quad,
+    winding normals into the open region."""
+    sc = trench_discs(width=width, depth=depth, flat=flat, ylen=ylen, spacing=spacing)
+    px, pz = sc["profile_x"], sc["profile_z"]
+    pn = sc["normals"][: px.size].astype(np.float64)
+    ny_rows = int(round(ylen / spacing)) + 1
+    ys = np.arange(ny_rows, dtype=np.float64) * spacing
+    P = px.size
+    verts = np.stack([np.tile(px, ny_rows), np.repeat(ys, P), np.tile(pz, ny_rows)], axis=1)
+    j, i = np.meshgrid(np.arange(ny_rows - 1), np.arange(P - 1), indexing="ij")
+    a = (j * P + i).ravel()
+    b = a + 1
+    c = a + P + 1
+    d = a + P
+    tris = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    seg = np.concatenate([i.ravel(), i.ravel()])
+    want = pn[seg] + pn[seg + 1]
+    want[:, 1] = 0.0
+    tris = _orient(verts, tris, want)
+    return {"kind": "triangles", "verts": verts.astype(np.float32), "tris": tris.astype(np.uint32),
+            "width": width, "depth": depth, "flat": flat, "ylen": ylen, "spacing": spacing}
+
+
 def hole_discs(radius=0.5, depth=5.0, half=1.0, spacing=0.02, disc_radius=None):
     """Cylindrical hole (axis z, floor z = 0, top z = depth) in a
     [-half, half]^2 cell, as an oriented-disc cloud with inward normals."""

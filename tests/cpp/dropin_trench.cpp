@@ -2,7 +2,7 @@
 // example uses its API (src/rti/main/test2.cpp:22-45): a constant-sticking
 // particle, diffuse reflection, points / normals / grid spacing setters.
 // Usage: dropin_trench <out.txt> [rays] [sticking] [cos_exponent]
-// Writes one line per disc: mc estimate, hit count.
+// Writes one line per disc: x y z nx ny nz radius mc-estimate hit-count.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -76,7 +76,10 @@ int main(int argc, char** argv) {
   auto cnt = dev.get_hit_cnts();
   FILE* f = std::fopen(argv[1], "w");
   if (!f) return 3;
-  for (size_t i = 0; i < est.size(); ++i) std::fprintf(f, "%.9g %zu\n", (double)est[i], cnt[i]);
+  for (size_t i = 0; i < est.size(); ++i)
+    std::fprintf(f, "%.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %zu\n", (double)points[i][0], (double)points[i][1],
+                 (double)points[i][2], (double)normals[i][0], (double)normals[i][1], (double)normals[i][2],
+                 (double)spacing[i], (double)est[i], cnt[i]);
   std::fclose(f);
   std::printf("discs %zu\n", est.size());
   return 0;
