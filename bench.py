@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark: traced ray segments per second of the Monte Carlo flux trace on
-one MI355X.
+one MI355X. Prints one JSON line.
 
-Workload (default, ``--workload hole``): a triangle-mesh cylindrical hole,
-radius 0.5, depth 5 (aspect 10), in a 2 x 2 periodic cell, 199,360
-triangles; sticking s in {0.01, 0.1, 1} x source exponent n in {1, 10}; each
-of the six (s, n) cases traces ``--rays`` primary rays (1e8 by default). One
-step traces all six cases as one persistent launch replayed from a HIP graph;
-inputs are resident in HBM before the timed region. The plain path (one
-thread per ray, one launch per case) is timed once beside it, and the CPU
-baseline (the oracle restatement of the reference tracer, OpenMP on the host
-cores) on a bounded sample of the same cases.
+Workloads (``--workload``):
+  hole   (default) triangle-mesh cylindrical hole, radius 0.5, depth 5
+         (aspect 10), 2 x 2 periodic cell, 199,360 triangles; sticking
+         s in {0.01, 0.1, 1} x source exponent n in {1, 10}; ``--rays``
+         primary rays per (s, n) case (1e8).
+  trench disc-cloud trench, width 1, depth 4, 0.02 grid (51,051 discs),
+         reflective x, periodic y, source at z = 4.1, cosine source,
+         s = 0.1, 1e6 rays (the reference's CPU-runnable case).
+  batch  64 disc-cloud trenches and holes (aspect 2-20, 1e4-5e5 discs,
+         s log-uniform 1e-3..1, diffuse or specular), ray budgets
+         proportional to area; ``--rays`` is the batch total (6.4e7 here;
+         the generator's own budgets sum to 6.4e8).
+  rough  periodic rough surface over 16 holes of aspect 6 (about 4M discs),
+         70/30 diffuse/specular, s = 0.02, ``--rays`` (1e9); also reports
+         64-bit fixed point vs FP64 atomic accumulation on the same paths.
 
-Prints one JSON line.
+A step traces the whole workload as one persistent launch replayed from a
+HIP graph, inputs resident in HBM, launch knobs from the autotuner
+(rti_amd.tune). The plain path (one thread per ray, one launch per
+geometry) is timed once beside it, and the CPU baseline (the oracle
+restatement of the reference tracer, OpenMP on the host cores) on a bounded
+sample of the same cases.
 """
 import argparse
 import json
@@ -29,6 +40,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 S_VALUES = (0.01, 0.1, 1.0)
 N_VALUES = (1.0, 10.0)
+METRIC = "ray segments/sec @1 MI355X, batched vs plain GPU vs CPU; flux err vs radiosity"
 
 
 def log(*a):
@@ -43,6 +55,40 @@ def hole_cases(rays):
         for n in N_VALUES:
             cases.append(dict(scene=sc, n_rays=int(rays), sticking=s, cos_exponent=n, bc_x=1, bc_y=1, seed=2))
     return sc, cases
+
+
+def workload(name, rays):
+    from rti_amd import geometry
+    if name == "hole":
+        r = 1e8 if rays is None else rays
+        sc, cases = hole_cases(r)
+        desc = (f"triangle-mesh cylindrical hole r=0.5 depth 5 (AR 10), 2x2 periodic cell, {sc['tris'].shape[0]} "
+                f"triangles; s in {list(S_VALUES)} x n in {list(N_VALUES)}, {int(r)} rays each")
+        return cases, desc, 48.0
+    if name == "trench":
+        r = 1e6 if rays is None else rays
+        sc = geometry.trench_discs()
+        cases = [dict(scene=sc, n_rays=int(r), sticking=0.1, bc_x=0, bc_y=1, src_z=4.1, seed=1)]
+        desc = (f"disc-cloud trench w=1 d=4 (AR 4), 0.02 grid, {sc['xyzr'].shape[0]} discs, reflective x, "
+                f"periodic y, source z=4.1, cosine, s=0.1, {int(r)} rays")
+        return cases, desc, 32.0
+    if name == "batch":
+        r = 6.4e7 if rays is None else rays
+        inst = geometry.batch_instances(seed=3, count=64, total_rays=r)
+        cases = [dict(scene=i["scene"], n_rays=i["n_rays"], sticking=i["sticking"], diffuse_prob=i["diffuse_prob"],
+                      bc_x=1, bc_y=1, seed=3) for i in inst]
+        nd = sum(c["scene"]["xyzr"].shape[0] for c in cases)
+        desc = (f"64 disc-cloud trenches/holes (AR 2-20, {nd} discs total, s log-uniform 1e-3..1, diffuse or "
+                f"specular), {int(r)} rays total proportional to area")
+        return cases, desc, 32.0
+    if name == "rough":
+        r = 1e9 if rays is None else rays
+        sc = geometry.rough_surface_discs()
+        cases = [dict(scene=sc, n_rays=int(r), sticking=0.02, diffuse_prob=0.7, bc_x=1, bc_y=1, seed=5)]
+        desc = (f"rough periodic surface (4096^2 height field, 256 bumps) over 16 holes of AR 6, "
+                f"{sc['xyzr'].shape[0]} discs, 70/30 diffuse/specular, s=0.02, {int(r)} rays")
+        return cases, desc, 32.0
+    raise SystemExit(f"unknown workload {name}")
 
 
 def bytes_per_segment(counters, prim_bytes):
@@ -60,40 +106,46 @@ def bytes_per_segment(counters, prim_bytes):
 
 
 def cpu_baseline(cases, budget_s=20.0):
-    """Oracle (restated reference tracer, OpenMP) on a bounded ray sample."""
+    """Oracle (restated reference tracer, OpenMP) on a bounded ray sample:
+    a pilot of 2e4 rays per case sizes the sample to about budget_s seconds."""
     import oracle
     from rti_amd import geometry
     threads = os.cpu_count() or 1
-    segs = 0
-    secs = 0.0
-    sample = []
-    per_case = max(1000, int(2e4))
-    # calibrate: grow the sample until the budget is used (about budget_s seconds)
-    for c in cases:
-        box, sz = geometry.scene_box(c["scene"])
-        r = oracle.trace(c["scene"], n_rays=per_case, box=box, src_z=sz, sticking=c["sticking"],
-                         cos_exponent=c["cos_exponent"], bc_x=c["bc_x"], bc_y=c["bc_y"], seed=c["seed"],
-                         threads=threads)
-        segs += r["segments"]
-        secs += r["seconds"]
-        sample.append(per_case)
-    if secs < budget_s * 0.5:
-        scale = min(50.0, budget_s / max(secs, 1e-3))
-        segs = 0
-        secs = 0.0
-        sample = []
-        for c in cases:
-            nr = int(per_case * scale)
-            box, sz = geometry.scene_box(c["scene"])
-            r = oracle.trace(c["scene"], n_rays=nr, box=box, src_z=sz, sticking=c["sticking"],
-                             cos_exponent=c["cos_exponent"], bc_x=c["bc_x"], bc_y=c["bc_y"], seed=c["seed"],
-                             threads=threads)
+    if len(cases) > 8:
+        cases = cases[:: max(1, len(cases) // 8)]
+    boxes = [geometry.scene_box(c["scene"], c.get("src_z")) for c in cases]
+
+    def run(per_case):
+        segs, secs = 0, 0.0
+        for c, (box, sz) in zip(cases, boxes):
+            r = oracle.trace(c["scene"], n_rays=per_case, box=box, src_z=sz, sticking=c["sticking"],
+                             cos_exponent=c.get("cos_exponent", 1.0), diffuse_prob=c.get("diffuse_prob", 1.0),
+                             bc_x=c["bc_x"], bc_y=c["bc_y"], seed=c["seed"], threads=threads)
             segs += r["segments"]
             secs += r["seconds"]
-            sample.append(nr)
+        return segs, secs
+    pilot = 20000
+    segs, secs = run(pilot)
+    per_case = int(pilot * min(2000.0, max(1.0, budget_s / max(secs, 1e-3))))
+    if per_case > pilot:
+        segs, secs = run(per_case)
     return {"value": segs / secs, "unit": "segments/s", "cores": threads, "kind": "port",
-            "sample": f"oracle on {sum(sample)} rays ({sample[0]} per (s, n) case, same geometry), "
-                      f"{segs} segments in {secs:.1f} s"}
+            "sample": f"oracle (restated reference tracer, own BVH, OpenMP) on {per_case} rays for each of "
+                      f"{len(cases)} cases of the same geometry: {segs} segments in {secs:.1f} s"}
+
+
+def measured_traffic(name, segments_per_launch):
+    """HBM-side traffic per launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE of the trace kernel, each scaled by what the
+    counter reads for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("bytes_per_segment:"):
+                return float(line.split(":")[1].split()[0]) * segments_per_launch
+    return None
 
 
 def main():
@@ -101,29 +153,33 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--rays", type=float, default=1e8, help="primary rays per (s, n) case")
+    ap.add_argument("--workload", default="hole", choices=["hole", "trench", "batch", "rough"])
+    ap.add_argument("--rays", type=float, default=None, help="primary rays (per case for hole, total otherwise)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the trace shards by ray-id range with one int64 all-reduce "
               "(design only); run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from rti_amd import Context
-    sc, cases = hole_cases(args.rays)
-    log(f"scene: {sc['tris'].shape[0]} triangles, {len(cases)} cases x {int(args.rays)} rays")
+    from rti_amd import Context, tune
+    cases, desc, prim_bytes = workload(args.workload, args.rays)
+    log(f"{args.workload}: {desc}")
+    knobs = {}
+    if not args.no_tune:
+        t = tune.tune(cases, fraction=0.05, log=log)
+        knobs = tune.knobs(t)
+        log(f"tuned {knobs}")
     ctx = Context(0)
     ctx.batch_set(cases)
-    knobs = dict(block_size=args.block, lds_stack=args.lds)
     ctx.set_params(**knobs)
     t0 = time.time()
     ctx.prepare()
-    log(f"build {time.time() - t0:.2f} s (first call includes HIP init)")
+    log(f"build {time.time() - t0:.2f} s")
 
-    # warmup + timed replays of the captured graph
+    # warmup + timed replays of the captured graph (HIP events on the trace stream)
     if args.warmup:
         ctx.batch_replay(args.warmup)
     t0 = time.perf_counter()
@@ -133,10 +189,26 @@ def main():
     ms_per_step = st["trace_ms"]
     counters = ctx.counters()
     segments = int(counters[0])
+    rays_per_step = int(sum(c["n_rays"] for c in cases))
     log(f"persistent: {ms_per_step:.1f} ms/step (wall {1e3 * wall / args.steps:.1f}), {segments} segments/step")
     value = segments / (ms_per_step / 1e3)
 
-    # plain path, once
+    extra = {}
+    if args.workload == "rough":
+        # same paths, FP64 atomic accumulation: the fixed point's error
+        ctx.batch_run()
+        fixed = ctx.batch_flux(0)["raw"]
+        shift = ctx.batch_stats()["fixed_shift"]
+        ctx.set_params(accum=1, **knobs)
+        ctx.batch_run()
+        fp64 = ctx.batch_flux(0)["raw"]
+        nz = fp64 > 0
+        rel = np.abs(fixed[nz] - fp64[nz]) / fp64[nz]
+        extra["fixed_vs_fp64"] = {"max_rel": float(rel.max()), "mean_rel": float(rel.mean()),
+                                  "max_abs": float(np.abs(fixed - fp64).max()), "fixed_shift": shift}
+        log(f"fixed vs fp64: max rel {rel.max():.3e}, mean rel {rel.mean():.3e}")
+        ctx.set_params(**knobs)
+
     plain = None
     if not args.no_plain:
         ctx.set_params(kernel=1, **knobs)
@@ -145,6 +217,7 @@ def main():
         pc = ctx.counters()
         plain = {"segments_per_s": int(pc[0]) / (pst["trace_ms"] / 1e3), "ms_per_step": pst["trace_ms"]}
         log(f"plain: {pst['trace_ms']:.1f} ms/step")
+    ctx.close()
 
     # instrumented run for the algorithmic bytes, on 1/20 of the rays
     cnt_ctx = Context(0)
@@ -152,11 +225,11 @@ def main():
     cnt_ctx.batch_set(small)
     cnt_ctx.set_params(kernel=2, **knobs)
     cnt_ctx.batch_run()
-    B, parts = bytes_per_segment(cnt_ctx.counters(), 48.0)
+    B, parts = bytes_per_segment(cnt_ctx.counters(), prim_bytes)
     cnt_ctx.close()
     achieved = value * B / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, segments),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()}}
 
     cpu = None
@@ -165,7 +238,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} segments/s on {cpu['cores']} threads")
 
     out = {
-        "metric": "ray segments/sec @1 MI355X, batched vs plain GPU vs CPU; flux err vs radiosity",
+        "metric": METRIC,
         "value": value,
         "unit": "segments/s",
         "n_gpus": 1,
@@ -177,17 +250,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "triangle-mesh cylindrical hole r=0.5 depth 5 (AR 10), 2x2 periodic cell, "
-                               f"{sc['tris'].shape[0]} triangles; s in {list(S_VALUES)} x n in {list(N_VALUES)}, "
-                               f"{int(args.rays)} rays each; batched persistent launch, HIP graph",
-                   "rays_per_step": int(args.rays) * len(cases), "segments_per_step": segments},
+        "config": {"workload": desc + "; one persistent launch per step, HIP graph",
+                   "rays_per_step": rays_per_step, "segments_per_step": segments, "knobs": knobs},
         "plain": plain,
         "batched_vs_plain": (value / plain["segments_per_s"]) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
-    ctx.close()
+    out.update(extra)
     print(json.dumps(out))
 
 

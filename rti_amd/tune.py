@@ -1,0 +1,76 @@
+"""Autotuner for the persistent trace kernel's launch knobs.
+
+For a batch (the same item dicts ``Context.batch_set`` takes) it traces a
+reduced ray budget with each candidate and keeps the fastest, by coordinate
+descent: ``compact_lanes`` (lanes of a wave that must have finished their
+closest-hit query before the wave shades and refills), then the LDS stack
+depth, then the block size. Results are cached in a JSON file keyed by the
+batch signature, so a repeated workload is not re-tuned.
+"""
+import hashlib
+import json
+import os
+
+from ._lib import Context
+
+COMPACT = (24, 32, 40, 48, 56, 64)
+LDS = (8, 16, 32)
+BLOCK = (64, 128, 256)
+
+
+def signature(items):
+    h = hashlib.sha1()
+    for it in items:
+        sc = it["scene"]
+        arr = sc["xyzr"] if sc["kind"] == "discs" else sc["tris"]
+        h.update(f"{sc['kind']}:{arr.shape[0]}:{it.get('n_rays')}:{it.get('sticking')}:"
+                 f"{it.get('cos_exponent', 1.0)}:{it.get('diffuse_prob', 1.0)}".encode())
+    return h.hexdigest()[:16]
+
+
+def _rate(ctx, knobs):
+    ctx.set_params(**knobs)
+    ctx.batch_replay(1)          # capture + warm
+    ctx.batch_replay(2)
+    st = ctx.batch_stats()
+    return float(ctx.counters()[0]) / (st["trace_ms"] / 1e3)
+
+
+def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
+    """Best knobs {compact_lanes, lds_stack, block_size} and their rate."""
+    key = signature(items)
+    if cache and os.path.exists(cache):
+        with open(cache) as fh:
+            db = json.load(fh)
+        if key in db:
+            return db[key]
+    small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
+    best = {"compact_lanes": 56, "lds_stack": 16, "block_size": 256}
+    trials = []
+    with Context() as ctx:
+        ctx.batch_set(small)
+        ctx.prepare()
+        for name, values in (("compact_lanes", COMPACT), ("lds_stack", LDS), ("block_size", BLOCK)):
+            scores = {}
+            for v in values:
+                knobs = dict(best, **{name: v})
+                scores[v] = _rate(ctx, knobs)
+                trials.append({**knobs, "segments_per_s": scores[v]})
+                if log:
+                    log(f"tune {knobs} -> {scores[v]:.3e} segments/s")
+            best[name] = max(scores, key=scores.get)
+    out = dict(best, segments_per_s=max(t["segments_per_s"] for t in trials), trials=trials, signature=key)
+    if cache:
+        db = {}
+        if os.path.exists(cache):
+            with open(cache) as fh:
+                db = json.load(fh)
+        db[key] = out
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        with open(cache, "w") as fh:
+            json.dump(db, fh, indent=1)
+    return out
+
+
+def knobs(tuned):
+    return {k: tuned[k] for k in ("compact_lanes", "lds_stack", "block_size")}
