@@ -96,6 +96,8 @@ typedef struct rti_params {
   int32_t compact_lanes;  /* persistent kernel: lanes of a wave whose closest-hit query
                              must be finished before the wave shades and refills
                              (1..64; 0 = tuned default)                                */
+  int32_t bvh_width;      /* traversal tree: 0 = default (4), 4 = four-wide quantised,
+                             2 = binary LBVH                                           */
 } rti_params;
 
 typedef struct rti_stats {
@@ -113,6 +115,8 @@ typedef struct rti_stats {
   double post_ms;         /* finalisation + download                                         */
   float box[6];           /* boundary xmin, xmax, ymin, ymax, zmin, zmax                     */
   float src_z;
+  uint32_t wide_stack;    /* most stack entries a four-wide traversal can hold               */
+  uint64_t wide_nodes;    /* four-wide traversal-tree nodes                                  */
 } rti_stats;
 
 /* One geometry of a batch. Exactly one of (xyzr, nrm) or (verts, tris) is set. */
@@ -170,6 +174,15 @@ int rti_get_counters(rti_ctx* ctx, uint64_t out[8]);
  * the children are relative to this geometry. Any output may be NULL. */
 int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* perm, uint64_t* n_nodes,
                       float* prim_a);
+
+/* The four-wide traversal tree of the (single) geometry: 64-byte records
+ * { float org[3]; uint32 meta = e_x | e_y << 8 | e_z << 16 | nchild << 24;
+ *   uint32 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z, pad, pad; int32 child[4] },
+ * child c's box planes fmaf(byte c of the plane word, 2^(e - 127), org);
+ * child >= 0 is a node index relative to this geometry, < 0 is ~stored
+ * primitive position. Root is record 0. The reference's Embree BVH is
+ * internal to Embree and has no such export; this one serves validation. */
+int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64_t* n_nodes);
 
 /* Batch: many geometries in one persistent launch (captured in a HIP graph
  * and replayed by rti_batch_replay). Results per instance by index. */

@@ -32,7 +32,7 @@ EXPORTS = [
     "rti_default_params", "rti_ctx_create", "rti_ctx_destroy", "rti_last_error", "rti_set_discs",
     "rti_set_triangles", "rti_set_params", "rti_run", "rti_prepare", "rti_trace", "rti_finalize", "rti_get_flux",
     "rti_get_counts", "rti_get_areas", "rti_get_neighbors", "rti_get_stats", "rti_get_adjoint", "rti_get_counters",
-    "rti_get_hierarchy",
+    "rti_get_hierarchy", "rti_get_wide_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
     "rti_batch_get_stats",
 ]
@@ -52,6 +52,7 @@ class Params(ctypes.Structure):
         ("roulette", ctypes.c_int32), ("src_z", ctypes.c_float), ("kernel", ctypes.c_int32),
         ("adjoint", ctypes.c_int32), ("accum", ctypes.c_int32), ("block_size", ctypes.c_int32),
         ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
+        ("bvh_width", ctypes.c_int32),
     ]
 
 
@@ -61,7 +62,8 @@ class Stats(ctypes.Structure):
         ("surface_hits", ctypes.c_uint64), ("misses", ctypes.c_uint64), ("truncated", ctypes.c_uint64),
         ("n_edges", ctypes.c_uint64), ("bvh_depth", ctypes.c_uint32), ("fixed_shift", ctypes.c_int32),
         ("trace_ms", ctypes.c_double), ("build_ms", ctypes.c_double), ("post_ms", ctypes.c_double),
-        ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float),
+        ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float), ("wide_stack", ctypes.c_uint32),
+        ("wide_nodes", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -69,6 +71,11 @@ class Stats(ctypes.Structure):
         d["box"] = list(self.box)
         return d
 
+
+# Four-wide traversal-tree record (64 bytes): child c's box planes are
+# fmaf(byte c of the plane word, 2^(e - 127), org) with e the axis's byte of meta.
+WIDE_NODE = np.dtype([("org", "<f4", 3), ("meta", "<u4"), ("lo", "<u4", 3), ("hi", "<u4", 3), ("pad", "<u4", 2),
+                      ("child", "<i4", 4)])
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -120,6 +127,7 @@ def _declare(L):
     L.rti_get_adjoint.argtypes = [vp, _dp, _dp]
     L.rti_get_counters.argtypes = [vp, _u64p]
     L.rti_get_hierarchy.argtypes = [vp, c.c_void_p, c.c_uint64, _u32p, _u64p, _fp]
+    L.rti_get_wide_hierarchy.argtypes = [vp, c.c_void_p, c.c_uint64, _u64p]
     L.rti_batch_set.argtypes = [vp, c.POINTER(Instance), c.c_uint32]
     L.rti_batch_replay.argtypes = [vp, c.c_uint32]
     L.rti_batch_get_flux.argtypes = [vp, c.c_uint32, _fp, _dp, _dp]
@@ -256,6 +264,14 @@ class Context:
         self._check(lib().rti_get_hierarchy(self._h, nodes.ctypes.data, nodes.size, _ptr(perm, _u32p),
                                             ctypes.byref(nn), _ptr(pa, _fp)))
         return nodes, perm, pa
+
+    def wide_hierarchy(self):
+        """Structured array of the four-wide traversal tree (root first)."""
+        nn = ctypes.c_uint64(0)
+        self._check(lib().rti_get_wide_hierarchy(self._h, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(int(nn.value), dtype=WIDE_NODE)
+        self._check(lib().rti_get_wide_hierarchy(self._h, nodes.ctypes.data, nodes.size, ctypes.byref(nn)))
+        return nodes
 
     def stats(self):
         s = Stats()

@@ -21,6 +21,7 @@
 #include "internal.hpp"
 
 using rti_dev::bvh_node;
+using rti_dev::bvh4_node;
 using rti_dev::inst_desc;
 
 namespace {
@@ -72,6 +73,7 @@ struct instance {
   float src_z = 0;
   float rmax = 0;
   uint32_t depth = 0;
+  uint32_t n_wide = 0, wide_stack = 0;
   uint64_t n_edges = 0;
   int shift = 32;
   // results (input order)
@@ -95,7 +97,7 @@ struct rti_ctx {
   bool batch_mode = false;  // instances came from rti_batch_set (own physics parameters)
   bool have_results = false;
   // scene store
-  dbuf prim_a, prim_b, prim_c, perm, nodes, nb_ptr, nb_col, areas, desc;
+  dbuf prim_a, prim_b, prim_c, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
   dbuf s1, s2, cnt, ds, dn, queue, stats;
   // finalisation scratch
   dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj;
@@ -149,6 +151,8 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
     return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 8, 16, 24 or 32");
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
+  if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
   return RTI_OK;
 }
 
@@ -300,6 +304,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->prim_c.ensure(np * 16));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
+  HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
   HIP_TRY(ctx, ctx->nb_ptr.ensure((np + 1) * 4));
   HIP_TRY(ctx, ctx->areas.ensure(np * 4));
   HIP_TRY(ctx, ctx->s1.ensure(np * 8));
@@ -309,8 +314,8 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->stats.ensure(64));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
   rti_impl::store_ptrs st{ctx->prim_a.as<float4>(), ctx->prim_b.as<float4>(), ctx->prim_c.as<float4>(),
-                          ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nb_ptr.as<uint32_t>(),
-                          nullptr, ctx->areas.as<float>()};
+                          ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
+                          ctx->nb_ptr.as<uint32_t>(), nullptr, ctx->areas.as<float>()};
   HIP_TRY(ctx, hipError_t(rti_impl::scratch_reserve(ctx->scratch, (uint32_t)maxn)));
   dbuf in_a, in_b;
   std::vector<rti_impl::build_job> jobs(ctx->inst.size());
@@ -348,6 +353,7 @@ int prepare(rti_ctx* ctx) {
     HIP_TRY(ctx, rti_impl::build_instance(J, st, ctx->scratch, &depth, ctx->stream));
     I.depth = depth;
     if (depth > 64) return fail(ctx, RTI_ERR_HIP, "hierarchy depth %u exceeds the traversal bound 64", depth);
+    HIP_TRY(ctx, rti_impl::build_wide(J, st, ctx->scratch, &I.n_wide, &I.wide_stack, ctx->stream));
     // neighbour degrees -> row offsets (absolute)
     uint32_t* rowp = ctx->nb_ptr.as<uint32_t>() + I.prim_begin;
     if (I.kind == 0) {
@@ -390,15 +396,34 @@ int prepare(rti_ctx* ctx) {
   return RTI_OK;
 }
 
-// Descriptors for the current per-instance parameters (cheap; done per trace).
+// Expected front-face hits of one path: the weight after k hits is (1-s)^k
+// and rejection control starts below 0.1 (rejection_control.hpp:27-61).
+double path_cost(const rti_params& p) {
+  const double s = p.sticking;
+  if (s >= 1.0) return 1.0;
+  return std::min(1e4, std::log(0.1) / std::log1p(-s) + 1.0);
+}
+
+// Descriptors for the current per-instance parameters (cheap; done per
+// trace). The global ray queue holds the instances' ray ranges in order of
+// decreasing expected path length, so the longest paths start first and the
+// launch does not end on a tail of long paths. The order does not change any
+// result: an instance's paths depend on its own key and ray ids only.
 int upload_descriptors(rti_ctx* ctx) {
-  std::vector<inst_desc> D(ctx->inst.size());
+  const size_t n = ctx->inst.size();
+  std::vector<size_t> order(n);
+  for (size_t k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return path_cost(ctx->inst[a].params) > path_cost(ctx->inst[b].params);
+  });
+  std::vector<inst_desc> D(n);
   uint64_t rb = 0;
-  for (size_t k = 0; k < ctx->inst.size(); ++k) {
+  for (size_t j = 0; j < n; ++j) {
+    const size_t k = order[j];
     instance& I = ctx->inst[k];
     derive_box(I);  // source height may have changed with the parameters
     const rti_params& p = I.params;
-    inst_desc& d = D[k];
+    inst_desc& d = D[j];
     std::memset(&d, 0, sizeof d);
     d.xmin = I.box[0];
     d.xmax = I.box[1];
@@ -417,6 +442,7 @@ int upload_descriptors(rti_ctx* ctx) {
     d.bc_y = p.bc_y;
     d.prim_kind = I.kind;
     d.root = (int32_t)I.node_begin;
+    d.root4 = (int32_t)I.node_begin;
     d.ray_begin = rb;
     d.n_rays = p.n_rays;
     d.seed = p.seed;
@@ -438,15 +464,23 @@ int upload_descriptors(rti_ctx* ctx) {
 }
 
 struct kernel_cfg {
-  int block, lds, grid;
+  int block, lds, grid, width;
 };
 
 int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
   const rti_params& p = ctx->params;
   cfg.block = p.block_size ? p.block_size : 256;
-  cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack();
+  cfg.width = p.bvh_width ? p.bvh_width : 4;
+  cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width);
+  if (cfg.width == 4) {
+    for (auto& I : ctx->inst)
+      if ((int)I.wide_stack > rti_impl::wide_stack_capacity())
+        return fail(ctx, RTI_ERR_INVALID_ARG,
+                    "four-wide traversal needs %u stack entries (capacity %d): use bvh_width = 2", I.wide_stack,
+                    rti_impl::wide_stack_capacity());
+  }
   int per_cu = 0;
-  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, adj, fp64, &per_cu));
+  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, adj, fp64, cfg.width, &per_cu));
   if (per_cu < 1) per_cu = 1;
   if (p.waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, p.waves_per_cu * 64 / cfg.block));
   cfg.grid = per_cu * ctx->num_cus;
@@ -480,6 +514,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.prim_b = ctx->prim_b.as<float4>();
   L.prim_c = ctx->prim_c.as<float4>();
   L.nodes = ctx->nodes.as<bvh_node>();
+  L.nodes4 = ctx->nodes4.as<bvh4_node>();
   L.nb_ptr = ctx->nb_ptr.as<uint32_t>();
   L.nb_col = ctx->nb_col.as<uint32_t>();
   L.acc_s1 = ctx->s1.as<unsigned long long>();
@@ -497,6 +532,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.grid_blocks = cfg.grid;
   L.count = count ? 1 : 0;
   L.compact = ctx->params.compact_lanes;
+  L.width = cfg.width;
   if (kernel == RTI_KERNEL_PERSISTENT) {
     L.ray_lo = 0;
     L.ray_hi = ctx->rays_total;
@@ -648,12 +684,15 @@ int set_single(rti_ctx* ctx, instance&& I) {
 void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   std::memset(out, 0, sizeof *out);
   uint64_t np = 0, nr = 0, ne = 0;
-  uint32_t depth = 0;
+  uint32_t depth = 0, wstack = 0;
+  uint64_t nw = 0;
   for (auto& J : ctx->inst) {
     np += J.n;
     nr += J.params.n_rays;
     ne += J.n_edges;
     depth = std::max(depth, J.depth);
+    wstack = std::max(wstack, J.wide_stack);
+    nw += J.n_wide;
   }
   out->n_prims = I ? I->n : np;
   out->n_rays = I ? I->params.n_rays : nr;
@@ -670,6 +709,8 @@ void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   out->post_ms = ctx->post_ms;
   std::memcpy(out->box, J.box, sizeof out->box);
   out->src_z = J.src_z;
+  out->wide_stack = I ? I->wide_stack : wstack;
+  out->wide_nodes = I ? I->n_wide : nw;
 }
 
 }  // namespace
@@ -893,6 +934,22 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
   if (perm) HIP_TRY(ctx, hipMemcpy(perm, ctx->perm.as<uint32_t>() + I.prim_begin, I.n * 4, hipMemcpyDeviceToHost));
   if (prim_a)
     HIP_TRY(ctx, hipMemcpy(prim_a, ctx->prim_a.as<float4>() + I.prim_begin, I.n * 16, hipMemcpyDeviceToHost));
+  return RTI_OK;
+}
+
+int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64_t* n_nodes) {
+  if (!ctx || !n_nodes) return fail(ctx, RTI_ERR_INVALID_ARG, "null argument");
+  if (ctx->inst.empty()) return fail(ctx, RTI_ERR_NO_GEOMETRY, "no geometry");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = prepare(ctx);
+  if (rc) return rc;
+  const instance& I = ctx->inst[0];
+  *n_nodes = I.n_wide;
+  if (nodes) {
+    if (cap_nodes < I.n_wide) return fail(ctx, RTI_ERR_INVALID_ARG, "node capacity too small");
+    HIP_TRY(ctx, hipMemcpy(nodes, ctx->nodes4.as<bvh4_node>() + I.node_begin, I.n_wide * sizeof(bvh4_node),
+                           hipMemcpyDeviceToHost));
+  }
   return RTI_OK;
 }
 

@@ -33,6 +33,7 @@ struct store_ptrs {
   float4* prim_c;
   uint32_t* perm;           // stored position -> input id (per instance)
   rti_dev::bvh_node* nodes;
+  rti_dev::bvh4_node* nodes4;  // four-wide traversal tree, same slot ranges as nodes
   uint32_t* nb_ptr;         // n_total + 1
   uint32_t* nb_col;
   float* areas;             // per stored position
@@ -60,6 +61,11 @@ void scratch_free(build_scratch& s);
 // Neighbour CSR is built separately because its size is data dependent.
 hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* depth_out,
                           hipStream_t stream);
+// Four-wide traversal tree collapsed from the instance's binary tree, into
+// nodes4[node_begin ..). n_wide: nodes written; stack_bound: the most stack
+// entries a closest-hit traversal of it can hold.
+hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* n_wide,
+                      uint32_t* stack_bound, hipStream_t stream);
 // Degrees of the overlap graph (discs), into s.counts[0..n).
 hipError_t neighbor_count(const build_job& job, const store_ptrs& st, build_scratch& s, hipStream_t stream);
 // Columns, with st.nb_ptr already holding absolute row offsets.
@@ -77,6 +83,7 @@ struct trace_launch {
   const float4* prim_b;
   const float4* prim_c;
   const rti_dev::bvh_node* nodes;
+  const rti_dev::bvh4_node* nodes4;
   const uint32_t* nb_ptr;
   const uint32_t* nb_col;
   unsigned long long* acc_s1;       // fixed point (or double bits in FP64 mode)
@@ -95,10 +102,14 @@ struct trace_launch {
   int grid_blocks;                  // persistent grid
   int count;                        // 1 = instrumented run (node / prim / neighbour counters)
   int compact;                      // persistent: lanes done before a wave shades (1..64, 0 = default)
+  int width;                        // traversal tree: 2 binary, 4 four-wide
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
-hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int* blocks_per_cu);
-int default_lds_stack();
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int width,
+                           int* blocks_per_cu);
+int default_lds_stack(int width);
+// Stack entries (LDS + private) a four-wide traversal holds at most.
+int wide_stack_capacity();
 int default_compact_lanes();
 
 // ---- post ----

@@ -6,6 +6,7 @@
 //   prim_b[p]  float4   discs: unit normal xyz, 0;    triangles: e1 = v1 - v0
 //   prim_c[p]  float4   triangles: e2 = v2 - v0 (unused for discs)
 //   nodes[i]   bvh_node 64 B, both children's boxes in the parent
+//   nodes4[i]  bvh4_node 64 B, four quantised child boxes (the traversal tree)
 //   nb_ptr/nb_col       neighbour CSR over absolute prim ids (discs)
 //   acc_*[p]            64-bit accumulators
 // Primitives are stored in Morton order of their instance, so neighbouring
@@ -30,6 +31,23 @@ struct __align__(16) bvh_node {
 };
 static_assert(sizeof(bvh_node) == 64, "node must be one 64-byte record");
 
+// Four-wide node, collapsed from the binary tree, with its children's boxes
+// quantised to 8 bits per plane in the frame of the node's own box:
+//   plane = fmaf(q, 2^(e_axis - 127), org_axis),
+// where q is byte c (child c) of the plane's word and e_axis the biased
+// exponent in `meta`. The build rounds lower planes down and upper planes up
+// through exactly this expression, so every quantised box contains the
+// child's exact box. Children 0..nchild-1 are valid; child[c] >= 0 is a
+// four-wide node index, child[c] < 0 is leaf ~prim. 64 bytes.
+struct __align__(16) bvh4_node {
+  float ox, oy, oz;
+  uint32_t meta;  // e_x | e_y << 8 | e_z << 16 | nchild << 24
+  uint32_t lo_x, lo_y, lo_z, hi_x;
+  uint32_t hi_y, hi_z, pad0, pad1;
+  int4 child;
+};
+static_assert(sizeof(bvh4_node) == 64, "four-wide node must be one 64-byte record");
+
 // Per-instance description, read by each lane for the instance its ray
 // belongs to. 128 bytes.
 struct __align__(16) inst_desc {
@@ -40,7 +58,8 @@ struct __align__(16) inst_desc {
   uint64_t ray_begin, n_rays;       // slice of the global ray queue
   uint32_t seed, key1, roulette, prim_begin;
   uint32_t n_prims, pad0, pad1, pad2;
-  float score_c, fx_scale, pad4, pad5;  // score_c = 1 / (n + 1); fixed-point scale 2^shift
+  float score_c, fx_scale;          // score_c = 1 / (n + 1); fixed-point scale 2^shift
+  int32_t root4, pad5;              // root4: absolute four-wide node index
 };
 static_assert(sizeof(inst_desc) == 128, "descriptor size");
 

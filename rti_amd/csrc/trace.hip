@@ -12,10 +12,14 @@
 //     ray queue with one atomic per wave (ballot + popcount give each idle
 //     lane its id), so lanes never wait for the longest path of a batch;
 //   * plain: one thread per ray, traced to termination (the baseline).
-// Traversal: binary LBVH, both child boxes in the parent (one 64-byte node
-// load per visit), near child first, far child pushed on a per-lane stack
-// whose top LS entries live in LDS and the rest (at most 64 - LS, the tree
-// depth bound) in private memory.
+// Traversal (closest hit), on one of two trees built from the same LBVH:
+//   four-wide (default): one 64-byte node holds four children's boxes,
+//     quantised to 8 bits per plane; hit children are visited nearest first
+//     and the others pushed with their entry distance, so a popped entry that
+//     lies beyond the closest hit found meanwhile is dropped without a fetch;
+//   binary: both child boxes in the parent (one 64-byte node per visit).
+// The per-lane stack keeps its top LS entries in LDS and the rest in private
+// memory.
 // Accumulation: per-primitive 64-bit integer atomics of round(v * 2^shift),
 // so totals are independent of the order of deposits (bit-exact run to run).
 
@@ -32,6 +36,9 @@ namespace {
 
 constexpr uint32_t kMaxSegmentsPerRay = 1u << 20;
 constexpr int kMaxBlock = 256;
+// Stack entries of the four-wide traversal (LDS + private). The build
+// reports the exact bound for each tree; a launch checks it.
+constexpr int kWideStack = 64;
 
 struct tparams {
   const inst_desc* desc;
@@ -41,6 +48,7 @@ struct tparams {
   const float4* __restrict__ pb;
   const float4* __restrict__ pc;
   const bvh_node* __restrict__ nodes;
+  const bvh4_node* __restrict__ nodes4;
   const uint32_t* __restrict__ nb_ptr;
   const uint32_t* __restrict__ nb_col;
   unsigned long long* s1;
@@ -73,6 +81,46 @@ struct trav_state {
   bool found;
 };
 
+// Closest-hit update for one primitive (disc or triangle) in [tnear, tbest];
+// equal distances go to the lower primitive id.
+template <bool COUNT>
+__device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3 d, uint32_t p, trav_state& T,
+                                          counters& C) {
+  if (COUNT) C.prims++;
+  const float4 A = P.pa[p];
+  const float4 B = P.pb[p];
+  float th;
+  bool ok;
+  if (A.w > 0.0f) {
+    // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
+    // [tnear, tfar], no back-face culling, |hit - centre| < radius)
+    const f3 n = make(B.x, B.y, B.z);
+    const float dn = dot(d, n);
+    const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
+    th = dot(co, n) / dn;
+    const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
+    ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
+  } else {
+    // triangle: Moller-Trumbore
+    const float4 C2 = P.pc[p];
+    const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
+    const f3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    const float idet = 1.0f / det;
+    const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
+    const float u = dot(tv, pv) * idet;
+    const f3 qv = cross(tv, e1);
+    const float v = dot(d, qv) * idet;
+    th = dot(e2, qv) * idet;
+    ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= T.tnear && th <= T.tbest;
+  }
+  if (ok && (th < T.tbest || !T.found || p < T.pbest)) {
+    T.tbest = th;
+    T.pbest = p;
+    T.found = true;
+  }
+}
+
 template <int LS, bool COUNT>
 __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint32_t* lds,
                                           uint32_t* ovf, counters& C) {
@@ -101,40 +149,7 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
     int cc = c == 0 ? ch.x : ch.y;
     if (!(h && cc < 0)) continue;
     if (c == 0) h0 = false; else h1 = false;
-    uint32_t p = (uint32_t)(~cc);
-    if (COUNT) C.prims++;
-    const float4 A = P.pa[p];
-    const float4 B = P.pb[p];
-    float th;
-    bool ok;
-    if (A.w > 0.0f) {
-      // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
-      // [tnear, tfar], no back-face culling, |hit - centre| < radius)
-      const f3 n = make(B.x, B.y, B.z);
-      const float dn = dot(d, n);
-      const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
-      th = dot(co, n) / dn;
-      const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
-      ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
-    } else {
-      // triangle: Moller-Trumbore
-      const float4 C2 = P.pc[p];
-      const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
-      const f3 pv = cross(d, e2);
-      const float det = dot(e1, pv);
-      const float idet = 1.0f / det;
-      const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
-      const float u = dot(tv, pv) * idet;
-      const f3 qv = cross(tv, e1);
-      const float v = dot(d, qv) * idet;
-      th = dot(e2, qv) * idet;
-      ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= T.tnear && th <= T.tbest;
-    }
-    if (ok && (th < T.tbest || !T.found || p < T.pbest)) {
-      T.tbest = th;
-      T.pbest = p;
-      T.found = true;
-    }
+    test_prim<COUNT>(P, o, d, (uint32_t)(~cc), T, C);
   }
   if (h0 && h1) {
     const bool near0 = tmin0 <= tmin1;
@@ -158,6 +173,154 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
   T.cur = (int)(T.sp < LS ? lds[T.sp * stride] : ovf[T.sp - LS]);
   return false;
 }
+
+// Dynamic LDS of the trace kernels: the per-lane stacks, entry j of thread t
+// at word/pair j * blockDim.x + t (a wave's lanes hit distinct banks).
+// Addresses are formed from this symbol at each use so that the accesses
+// stay ds_read / ds_write.
+extern __shared__ uint32_t trace_lds[];
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+
+// Stack slot j of this thread, typed as LDS so that a conditional access to
+// either the LDS part or the private part cannot be merged into one generic
+// (flat) access.
+__device__ __forceinline__ lds_u64* lds_slot(uint32_t j) {
+  return (lds_u64*)(trace_lds) + j * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ uint32_t sel4(int4 v, uint32_t c) {
+  const int lo = (c & 1u) ? v.y : v.x;
+  const int hi = (c & 1u) ? v.w : v.z;
+  return (uint32_t)((c & 2u) ? hi : lo);
+}
+
+// tmin >= tnear > 0, so the float's bits order like the float; the low two
+// bits carry the child slot (ties go to the lower slot).
+__device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
+  return hit ? ((__float_as_uint(tmin) & ~3u) | c) : 0xffffffffu;
+}
+
+// One four-wide node per call. Leaf children are tested right away; hit
+// internal children are ordered by entry distance, the nearest is visited
+// next and the rest are pushed farthest first, each with its entry distance,
+// so that a popped entry beyond the closest hit found meanwhile is dropped
+// without fetching it. Returns true when the query is finished.
+//
+// Slab distances: the plane of byte q is org + q * 2^e, and
+//   t = (org + q * 2^e) * inv - o * inv = q * (2^e * inv) + (org * inv - o * inv),
+// one fma per plane with the two per-node factors.
+template <int LS, bool COUNT>
+__device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint64_t* ovf,
+                                           counters& C) {
+  const uint4* q = reinterpret_cast<const uint4*>(P.nodes4 + T.cur);
+  const uint4 q0 = q[0], q1 = q[1], q2 = q[2];
+  const int4 ch = reinterpret_cast<const int4*>(q)[3];
+  if (COUNT) C.nodes++;
+  const f3 inv = T.inv, oi = T.oi;
+  const uint32_t meta = q0.w;
+  const float ax = __uint_as_float((meta & 0xffu) << 23) * inv.x;
+  const float ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * inv.y;
+  const float az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * inv.z;
+  const float bx = fmaf(__uint_as_float(q0.x), inv.x, -oi.x);
+  const float by = fmaf(__uint_as_float(q0.y), inv.y, -oi.y);
+  const float bz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
+  const uint32_t m = meta >> 24;
+  // entry / exit planes per axis follow the sign of the direction
+  const uint32_t nx = inv.x >= 0.0f ? q1.x : q1.w, fx = inv.x >= 0.0f ? q1.w : q1.x;
+  const uint32_t ny = inv.y >= 0.0f ? q1.y : q2.x, fy = inv.y >= 0.0f ? q2.x : q1.y;
+  const uint32_t nz = inv.z >= 0.0f ? q1.z : q2.y, fz = inv.z >= 0.0f ? q2.y : q1.z;
+  float tmin[4];
+  bool hit[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int sh = 8 * c;
+    const float tnx = fmaf((float)((nx >> sh) & 0xffu), ax, bx);
+    const float tny = fmaf((float)((ny >> sh) & 0xffu), ay, by);
+    const float tnz = fmaf((float)((nz >> sh) & 0xffu), az, bz);
+    const float tfx = fmaf((float)((fx >> sh) & 0xffu), ax, bx);
+    const float tfy = fmaf((float)((fy >> sh) & 0xffu), ay, by);
+    const float tfz = fmaf((float)((fz >> sh) & 0xffu), az, bz);
+    tmin[c] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, T.tnear));
+    const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, T.tbest));
+    hit[c] = (uint32_t)c < m && tmin[c] <= tmax;
+  }
+  uint32_t leaves = 0;
+  leaves |= (hit[0] && ch.x < 0) ? 1u : 0u;
+  leaves |= (hit[1] && ch.y < 0) ? 2u : 0u;
+  leaves |= (hit[2] && ch.z < 0) ? 4u : 0u;
+  leaves |= (hit[3] && ch.w < 0) ? 8u : 0u;
+  while (leaves) {
+    const uint32_t c = (uint32_t)(__ffs(leaves) - 1);
+    leaves &= leaves - 1;
+    test_prim<COUNT>(P, o, d, ~sel4(ch, c), T, C);
+  }
+  uint32_t k0 = sort_key(hit[0] && ch.x >= 0 && tmin[0] <= T.tbest, tmin[0], 0);
+  uint32_t k1 = sort_key(hit[1] && ch.y >= 0 && tmin[1] <= T.tbest, tmin[1], 1);
+  uint32_t k2 = sort_key(hit[2] && ch.z >= 0 && tmin[2] <= T.tbest, tmin[2], 2);
+  uint32_t k3 = sort_key(hit[3] && ch.w >= 0 && tmin[3] <= T.tbest, tmin[3], 3);
+  {
+    uint32_t a = min(k0, k1), b = max(k0, k1);
+    k0 = a; k1 = b;
+    a = min(k2, k3); b = max(k2, k3);
+    k2 = a; k3 = b;
+    a = min(k0, k2); b = max(k0, k2);
+    k0 = a; k2 = b;
+    a = min(k1, k3); b = max(k1, k3);
+    k1 = a; k3 = b;
+    a = min(k1, k2); b = max(k1, k2);
+    k1 = a; k2 = b;
+  }
+  if (k0 != 0xffffffffu) {
+    const uint32_t ks[3] = {k3, k2, k1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (ks[i] == 0xffffffffu) continue;
+      // entry: node in the low word, entry distance bits in the high word
+      const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | sel4(ch, ks[i] & 3u);
+      if (T.sp < LS) *lds_slot((uint32_t)T.sp) = e;
+      else ovf[T.sp - LS] = e;
+      ++T.sp;
+    }
+    T.cur = (int)sel4(ch, k0 & 3u);
+    return false;
+  }
+  while (T.sp > 0) {
+    --T.sp;
+    // the private read is volatile so that the two reads are not merged into
+    // one generic (flat) access through a selected pointer
+    uint64_t e;
+    if (T.sp < LS) e = *lds_slot((uint32_t)T.sp);
+    else e = const_cast<const volatile uint64_t*>(ovf)[T.sp - LS];
+    if (__uint_as_float((uint32_t)(e >> 32)) <= T.tbest) {
+      T.cur = (int)(uint32_t)e;
+      return false;
+    }
+  }
+  return true;
+}
+
+// Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
+// private array.
+template <int LS, int W>
+struct lane_stack;
+
+template <int LS>
+struct lane_stack<LS, 2> {
+  uint32_t ovf[64 - LS];
+  template <bool COUNT>
+  __device__ __forceinline__ bool step(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
+    return trav_step<LS, COUNT>(P, o, d, T, trace_lds + threadIdx.x, ovf, C);
+  }
+};
+
+template <int LS>
+struct lane_stack<LS, 4> {
+  uint64_t ovf[kWideStack - LS];
+  template <bool COUNT>
+  __device__ __forceinline__ bool step(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
+    return trav4_step<LS, COUNT>(P, o, d, T, ovf, C);
+  }
+};
 
 // Lateral boundary faces x = xmin/xmax, y = ymin/ymax over z in [zmin, zmax]
 // (src/rti/geo/boundary_x_y.hpp:147-235; no top or bottom face).
@@ -283,6 +446,7 @@ __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_st
 
 // Start of a segment: the boundary-face distance bounds the closest-hit
 // query, which starts at the instance's root.
+template <int W>
 __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C) {
   const inst_desc& D = P.desc[R.inst];
   C.seg++;
@@ -292,7 +456,7 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.tbest = T.tb;
   T.found = false;
   T.pbest = 0;
-  T.cur = D.root;
+  T.cur = W == 4 ? D.root4 : D.root;
   T.sp = 0;
   const float tiny = 1e-20f;
   T.inv.x = 1.0f / (fabsf(R.d.x) > tiny ? R.d.x : copysignf(tiny, R.d.x));
@@ -302,15 +466,16 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
 }
 
 // The consequences of a finished closest-hit query (tracer.hpp:184-253).
-// Returns false when the ray terminates.
+// Returns false when the ray terminates (the per-ray segment cap is applied
+// by the caller).
 template <bool ADJ, bool FP64ACC, bool COUNT>
-__device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, const trav_state& T, counters& C) {
+__device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, const trav_state& T, counters& C) {
   const inst_desc& D = P.desc[R.inst];
+  // each counter is updated at one place, by straight-line arithmetic, so
+  // the counters stay in registers
+  C.miss += (!T.found && T.face < 0) ? 1u : 0u;
   if (!T.found) {
-    if (T.face < 0) {  // tracer.hpp:184-190
-      C.miss++;
-      return false;
-    }
+    if (T.face < 0) return false;  // tracer.hpp:184-190
     // boundary_x_y::process_hit (src/rti/geo/boundary_x_y.hpp:91-143)
     const f3 h = R.o + R.d * T.tb;
     if (T.face == 0 || T.face == 2) {
@@ -320,7 +485,7 @@ __device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, c
       if (D.bc_y == 0) { R.d.y = -R.d.y; R.o = h; }
       else R.o = make(h.x, T.face == 1 ? D.ymax : D.ymin, h.z);
     }
-    return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+    return true;
   }
   const uint32_t prim = T.pbest;
   const float tp = T.tbest;
@@ -335,7 +500,7 @@ __device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, c
   }
   if (dot(R.d, n) > 0.0f) {  // back face: pass through (tracer.hpp:212-227)
     R.o = R.o + R.d * tp;
-    return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+    return true;
   }
   C.hit++;
   // stick and deposit (tracer.hpp:228-237)
@@ -373,33 +538,51 @@ __device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, c
   R.o = R.o + R.d * tp;  // meta_geometry::get_new_origin (src/rti/geo/meta_geometry.hpp:17-23)
   if (u01(u.w) < D.diffuse_prob) R.d = diffuse_dir(n, u01(u.y), u01(u.z));
   else R.d = R.d - n * (2.0f * dot(R.d, n));  // specular::use (src/rti/reflection/specular.hpp:16-40)
-  return R.segs < kMaxSegmentsPerRay ? true : (C.trunc++, false);
+  return true;
 }
 
-__device__ __forceinline__ void flush_counters(const tparams& P, const counters& C, bool count) {
-  // wave reduction, then one atomic per wave
-  uint32_t v[8] = {C.seg, C.hit, C.miss, C.trunc, C.nodes, C.prims, C.nbt, C.deps};
-  const int nv = count ? 8 : 4;
-  for (int i = 0; i < nv; ++i) {
-    unsigned long long x = v[i];
+template <bool ADJ, bool FP64ACC, bool COUNT>
+__device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, const trav_state& T, counters& C) {
+  const bool more = shade_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
+  const bool cap = more && R.segs >= kMaxSegmentsPerRay;
+  C.trunc += cap ? 1u : 0u;
+  return more && !cap;
+}
+
+// Wave reduction of the counters, then one atomic per wave and counter. The
+// counters stay in registers: every index below is a compile-time constant.
+__device__ __forceinline__ void flush_one(const tparams& P, int i, uint32_t v) {
+  unsigned long long x = v;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd(P.stats + i, x);
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(P.stats + i, x);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counters(const tparams& P, const counters& C) {
+  flush_one(P, 0, C.seg);
+  flush_one(P, 1, C.hit);
+  flush_one(P, 2, C.miss);
+  flush_one(P, 3, C.trunc);
+  if (COUNT) {
+    flush_one(P, 4, C.nodes);
+    flush_one(P, 5, C.prims);
+    flush_one(P, 6, C.nbt);
+    flush_one(P, 7, C.deps);
   }
 }
 
 // Persistent, wavefront-scheduled trace. Each wave holds up to 64 live rays
 // and alternates two phases:
 //   traverse: every lane with an open closest-hit query advances one node per
-//     iteration, until at least `compact` lanes have no open query;
+//     iteration, until at most a fraction (64 - compact) / 64 of the live
+//     lanes still have an open query;
 //   shade + refill: the lanes whose query finished process the hit and start
 //     their next segment; lanes whose ray ended take new rays from the global
 //     queue, one atomic per wave (ballot + popcount assign consecutive ids).
-template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
+template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
-  extern __shared__ uint32_t smem[];
-  uint32_t* lds = smem + threadIdx.x;
-  uint32_t ovf[64 - LS];
+  lane_stack<LS, W> S;
   counters C;
   ray_state R;
   trav_state T;
@@ -419,7 +602,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
         const unsigned long long my = base + __popcll(need & lt);
         if (my < P.ray_hi) {
           start_ray(P, my, R);
-          begin_segment(P, R, T, C);
+          begin_segment<W>(P, R, T, C);
           alive = true;
           tracing = true;
         } else {
@@ -427,28 +610,32 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_persistent(tparams P) {
         }
       }
     }
-    if (__ballot(alive) == 0ull) break;
+    const unsigned long long live = __ballot(alive);
+    if (live == 0ull) break;
+    // At most (64 - compact) / 64 of the live lanes may still be tracing when
+    // the wave shades. With a full wave that is 64 - compact lanes; while the
+    // wave drains (queue empty, fewer live lanes) the bound shrinks with it,
+    // so a few long paths still advance many nodes per shading pass.
+    const int still = ((64 - compact) * __popcll(live)) >> 6;
     for (;;) {
-      if (tracing) tracing = !trav_step<LS, COUNT>(P, R.o, R.d, T, lds, ovf, C);
-      if (__popcll(__ballot(!tracing)) >= compact) break;
+      if (tracing) tracing = !S.template step<COUNT>(P, R.o, R.d, T, C);
+      if (__popcll(__ballot(tracing)) <= still) break;
     }
     if (alive && !tracing) {
       alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
       if (alive) {
-        begin_segment(P, R, T, C);
+        begin_segment<W>(P, R, T, C);
         tracing = true;
       }
     }
   }
-  flush_counters(P, C, COUNT);
+  flush_counters<COUNT>(P, C);
 }
 
 // Plain trace: one thread per ray, every segment traversed to the end.
-template <int LS, bool ADJ, bool FP64ACC, bool COUNT>
+template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
-  extern __shared__ uint32_t smem[];
-  uint32_t* lds = smem + threadIdx.x;
-  uint32_t ovf[64 - LS];
+  lane_stack<LS, W> S;
   counters C;
   ray_state R;
   trav_state T;
@@ -456,32 +643,56 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   if (gid < P.ray_hi) {
     start_ray(P, gid, R);
     for (;;) {
-      begin_segment(P, R, T, C);
-      while (!trav_step<LS, COUNT>(P, R.o, R.d, T, lds, ovf, C)) {
+      begin_segment<W>(P, R, T, C);
+      while (!S.template step<COUNT>(P, R.o, R.d, T, C)) {
       }
       if (!finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C)) break;
     }
   }
-  flush_counters(P, C, COUNT);
+  flush_counters<COUNT>(P, C);
 }
 
 using kfn = void (*)(tparams);
 
-template <int LS>
+template <int LS, int W>
 kfn pick(bool persistent, bool adj, bool fp64, bool count) {
-  if (count) return persistent ? k_trace_persistent<LS, false, false, true> : k_trace_plain<LS, false, false, true>;
+  if (count)
+    return persistent ? k_trace_persistent<LS, W, false, false, true> : k_trace_plain<LS, W, false, false, true>;
   if (persistent) {
-    if (adj) return fp64 ? k_trace_persistent<LS, true, true, false> : k_trace_persistent<LS, true, false, false>;
-    return fp64 ? k_trace_persistent<LS, false, true, false> : k_trace_persistent<LS, false, false, false>;
+    if (adj) return fp64 ? k_trace_persistent<LS, W, true, true, false> : k_trace_persistent<LS, W, true, false, false>;
+    return fp64 ? k_trace_persistent<LS, W, false, true, false> : k_trace_persistent<LS, W, false, false, false>;
   }
-  if (adj) return fp64 ? k_trace_plain<LS, true, true, false> : k_trace_plain<LS, true, false, false>;
-  return fp64 ? k_trace_plain<LS, false, true, false> : k_trace_plain<LS, false, false, false>;
+  if (adj) return fp64 ? k_trace_plain<LS, W, true, true, false> : k_trace_plain<LS, W, true, false, false>;
+  return fp64 ? k_trace_plain<LS, W, false, true, false> : k_trace_plain<LS, W, false, false, false>;
+}
+
+template <int W>
+kfn pick_ls(int lds_stack, bool persistent, bool adj, bool fp64, bool count) {
+  switch (lds_stack) {
+    case 4: return pick<4, W>(persistent, adj, fp64, count);
+    case 8: return pick<8, W>(persistent, adj, fp64, count);
+    case 16: return pick<16, W>(persistent, adj, fp64, count);
+    case 24: return pick<24, W>(persistent, adj, fp64, count);
+    case 32: return pick<32, W>(persistent, adj, fp64, count);
+    default: return nullptr;
+  }
+}
+
+kfn pick_kernel(int width, int lds_stack, bool persistent, bool adj, bool fp64, bool count) {
+  if (width == 2) return pick_ls<2>(lds_stack, persistent, adj, fp64, count);
+  if (width == 4) return pick_ls<4>(lds_stack, persistent, adj, fp64, count);
+  return nullptr;
+}
+
+size_t stack_lds_bytes(int width, int block_size, int lds_stack) {
+  return (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
 }
 
 }  // namespace
 
-int default_lds_stack() { return 16; }
+int default_lds_stack(int width) { return width == 4 ? 8 : 16; }
 int default_compact_lanes() { return 56; }
+int wide_stack_capacity() { return kWideStack; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   tparams P;
@@ -493,6 +704,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.pb = L.prim_b;
   P.pc = L.prim_c;
   P.nodes = L.nodes;
+  P.nodes4 = L.nodes4;
   P.nb_ptr = L.nb_ptr;
   P.nb_col = L.nb_col;
   P.s1 = L.acc_s1;
@@ -507,16 +719,9 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   const bool count = L.count != 0;
   const int bs = L.block_size;
   if (bs < 64 || bs > kMaxBlock || (bs & 63)) return hipErrorInvalidValue;
-  kfn f;
-  switch (L.lds_stack) {
-    case 4: f = pick<4>(persistent, L.adjoint, L.accum_fp64, count); break;
-    case 8: f = pick<8>(persistent, L.adjoint, L.accum_fp64, count); break;
-    case 16: f = pick<16>(persistent, L.adjoint, L.accum_fp64, count); break;
-    case 24: f = pick<24>(persistent, L.adjoint, L.accum_fp64, count); break;
-    case 32: f = pick<32>(persistent, L.adjoint, L.accum_fp64, count); break;
-    default: return hipErrorInvalidValue;
-  }
-  const size_t shmem = (size_t)bs * L.lds_stack * 4;
+  const kfn f = pick_kernel(L.width, L.lds_stack, persistent, L.adjoint, L.accum_fp64, count);
+  if (!f) return hipErrorInvalidValue;
+  const size_t shmem = stack_lds_bytes(L.width, bs, L.lds_stack);
   uint64_t grid;
   if (persistent) {
     grid = (uint64_t)L.grid_blocks;
@@ -528,18 +733,12 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int* blocks_per_cu) {
-  kfn f;
-  switch (lds_stack) {
-    case 4: f = pick<4>(persistent, adjoint, fp64, false); break;
-    case 8: f = pick<8>(persistent, adjoint, fp64, false); break;
-    case 16: f = pick<16>(persistent, adjoint, fp64, false); break;
-    case 24: f = pick<24>(persistent, adjoint, fp64, false); break;
-    case 32: f = pick<32>(persistent, adjoint, fp64, false); break;
-    default: return hipErrorInvalidValue;
-  }
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int width,
+                           int* blocks_per_cu) {
+  const kfn f = pick_kernel(width, lds_stack, persistent, adjoint, fp64, false);
+  if (!f) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block_size,
-                                                      (size_t)block_size * lds_stack * 4);
+                                                      stack_lds_bytes(width, block_size, lds_stack));
 }
 
 }  // namespace rti_impl

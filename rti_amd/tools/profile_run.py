@@ -3,9 +3,10 @@
   python -m rti_amd.tools.profile_run copy  [--mib 1024]
       one device-to-device copy of a buffer of known size (torch's copy
       kernel): the calibration for FETCH_SIZE / WRITE_SIZE;
-  python -m rti_amd.tools.profile_run trace [--rays 1e7] [--launches 2]
+  python -m rti_amd.tools.profile_run trace [--rays 1e7] [--launches 2] [--knobs JSON]
       the benchmark workload (triangle hole, six (s, n) cases) traced as one
-      persistent launch per call, inputs resident.
+      persistent launch per call, inputs resident; --knobs sets kernel
+      parameters (e.g. '{"bvh_width": 2, "lds_stack": 16}').
 Prints the work done so a summary can turn counters into bytes per segment.
 """
 import argparse
@@ -23,6 +24,7 @@ def main():
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--rays", type=float, default=1e7)
     ap.add_argument("--launches", type=int, default=2)
+    ap.add_argument("--knobs", default="{}")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -39,6 +41,7 @@ def main():
     sc, cases = hole_cases(a.rays)
     ctx = Context(0)
     ctx.batch_set(cases)
+    ctx.set_params(**json.loads(a.knobs))
     ctx.prepare()
     segs = []
     for _ in range(a.launches):

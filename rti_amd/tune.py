@@ -13,9 +13,9 @@ import os
 
 from ._lib import Context
 
-COMPACT = (24, 32, 40, 48, 56, 64)
-LDS = (8, 16, 32)
-BLOCK = (64, 128, 256)
+COMPACT = (40, 48, 56, 60, 64)
+LDS = (4, 8, 16)
+BLOCK = (128, 256)
 
 
 def signature(items):
@@ -45,7 +45,7 @@ def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
         if key in db:
             return db[key]
     small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
-    best = {"compact_lanes": 56, "lds_stack": 16, "block_size": 256}
+    best = {"compact_lanes": 56, "lds_stack": 8, "block_size": 256}
     trials = []
     with Context() as ctx:
         ctx.batch_set(small)

@@ -1,0 +1,153 @@
+"""The four-wide traversal tree (quantised child boxes) and its traversal.
+
+* structure: every node reachable once, every primitive exactly one leaf,
+  1..4 children per node;
+* conservativeness: each child's box, decoded exactly as the kernel decodes it
+  (org + q * 2^(e - 127) rounded once to float32), encloses the exact bounds
+  of every primitive below it -- compared without tolerance;
+* the four-wide and the binary traversal return the same closest hits, so
+  whole runs agree bit for bit;
+* the wide tree visits fewer nodes per segment than the binary one.
+"""
+import numpy as np
+import pytest
+
+from rti_amd import Context, geometry, run_scene
+from tests.helpers import small_hole_mesh, small_trench
+
+pytestmark = pytest.mark.gpu
+
+
+def _prim_bounds(scene, perm):
+    if scene["kind"] == "discs":
+        x = scene["xyzr"][perm].astype(np.float64)
+        n = scene["normals"][perm].astype(np.float64)
+        ext = x[:, 3:4] * np.sqrt(np.maximum(0.0, 1.0 - n * n))
+        return x[:, :3] - ext, x[:, :3] + ext
+    v = scene["verts"].astype(np.float64)[scene["tris"][perm]]
+    return v.min(axis=1), v.max(axis=1)
+
+
+def _decode(node, c):
+    """Child c's box planes as the traversal computes them (float32)."""
+    lo = np.zeros(3)
+    hi = np.zeros(3)
+    for k in range(3):
+        e = (int(node["meta"]) >> (8 * k)) & 0xFF
+        s = np.float32(np.ldexp(1.0, e - 127))
+        ql = (int(node["lo"][k]) >> (8 * c)) & 0xFF
+        qh = (int(node["hi"][k]) >> (8 * c)) & 0xFF
+        # q * s is exact in float32; one rounding for the sum, as fmaf does
+        lo[k] = np.float32(node["org"][k]) + np.float32(ql) * s
+        hi[k] = np.float32(node["org"][k]) + np.float32(qh) * s
+    return lo, hi
+
+
+def _validate_wide(wide, perm, scene):
+    n = len(perm)
+    plo, phi = _prim_bounds(scene, perm)
+    seen_prim = np.zeros(n, dtype=np.int64)
+    seen_node = np.zeros(len(wide), dtype=np.int64)
+    order, stack = [], [0]
+    while stack:
+        v = stack.pop()
+        seen_node[v] += 1
+        order.append(v)
+        m = int(wide["meta"][v]) >> 24
+        assert 1 <= m <= 4, f"node {v} has {m} children"
+        for c in range(m):
+            ch = int(wide["child"][v][c])
+            if ch >= 0:
+                stack.append(ch)
+            else:
+                seen_prim[~ch] += 1
+    assert np.all(seen_node == 1), "every four-wide node is reached exactly once"
+    assert np.all(seen_prim == 1), f"{np.sum(seen_prim != 1)} primitives are not exactly one leaf"
+    sub_lo = np.full((len(wide), 3), np.inf)
+    sub_hi = np.full((len(wide), 3), -np.inf)
+    bad = 0
+    for v in reversed(order):
+        m = int(wide["meta"][v]) >> 24
+        for c in range(m):
+            ch = int(wide["child"][v][c])
+            blo, bhi = (plo[~ch], phi[~ch]) if ch < 0 else (sub_lo[ch], sub_hi[ch])
+            qlo, qhi = _decode(wide[v], c)
+            if np.any(qlo > blo) or np.any(qhi < bhi):
+                bad += 1
+                if bad < 4:
+                    print("node", v, "child", c, "->", ch, "box", qlo, qhi, "needs", blo, bhi)
+            sub_lo[v] = np.minimum(sub_lo[v], blo)
+            sub_hi[v] = np.maximum(sub_hi[v], bhi)
+    assert bad == 0, f"{bad} quantised child boxes do not enclose their subtree"
+
+
+def _one_disc():
+    return {"kind": "discs", "xyzr": np.array([[0.0, 0.0, 0.0, 0.5]], np.float32),
+            "normals": np.array([[0.0, 0.0, 1.0]], np.float32)}
+
+
+def _two_discs():
+    return {"kind": "discs", "xyzr": np.array([[-0.3, 0.0, 0.0, 0.25], [0.3, 0.0, 0.1, 0.25]], np.float32),
+            "normals": np.array([[0.0, 0.0, 1.0], [0.0, 0.6, 0.8]], np.float32)}
+
+
+SCENES = {
+    "one_disc": _one_disc,
+    "two_discs": _two_discs,
+    "small_trench": small_trench,
+    "hole_discs": lambda: geometry.hole_discs(spacing=0.05),
+    "hole_mesh": small_hole_mesh,
+    "trench_mesh": lambda: geometry.trench_triangles(flat=1.0, ylen=0.5, spacing=0.05),
+}
+
+
+@pytest.mark.parametrize("which", list(SCENES))
+def test_wide_tree_encloses_every_primitive(which):
+    sc = SCENES[which]()
+    n = sc["xyzr"].shape[0] if sc["kind"] == "discs" else sc["tris"].shape[0]
+    print(f"[gpu] wide tree {which} n={n}", flush=True)
+    with Context() as ctx:
+        ctx.set_scene(sc)
+        ctx.set_params(n_rays=16, sticking=1.0)
+        _, perm, _ = ctx.hierarchy()
+        wide = ctx.wide_hierarchy()
+        st = ctx.stats()
+    assert st["wide_nodes"] == len(wide)
+    assert len(wide) <= max(n - 1, 1)
+    assert 0 <= st["wide_stack"] <= 64
+    _validate_wide(wide, perm, sc)
+
+
+@pytest.mark.parametrize("which,kw", [
+    ("small_trench", dict(sticking=0.1, bc_x=0, bc_y=1)),
+    ("hole_discs", dict(sticking=0.05, bc_x=1, bc_y=1, diffuse_prob=0.6)),
+    ("hole_mesh", dict(sticking=0.1, bc_x=1, bc_y=1, cos_exponent=10.0)),
+    ("trench_mesh", dict(sticking=0.02, bc_x=0, bc_y=1)),
+])
+def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
+    sc = SCENES[which]()
+    print(f"[gpu] wide vs binary {which}", flush=True)
+    base = dict(n_rays=60_000, seed=11, **kw)
+    a = run_scene(sc, bvh_width=4, **base)
+    b = run_scene(sc, bvh_width=2, **base)
+    c = run_scene(sc, bvh_width=4, kernel=1, **base)
+    for other in (b, c):
+        assert np.array_equal(a["counts"], other["counts"])
+        assert np.array_equal(a["raw"], other["raw"])
+        assert a["stats"]["segments"] == other["stats"]["segments"]
+
+
+def test_wide_tree_visits_fewer_nodes():
+    sc = small_hole_mesh()
+    print("[gpu] wide vs binary node visits", flush=True)
+    kw = dict(n_rays=50_000, sticking=0.1, bc_x=1, bc_y=1, kernel=2, seed=3)
+    a = run_scene(sc, bvh_width=4, **kw)["counters"]
+    b = run_scene(sc, bvh_width=2, **kw)["counters"]
+    assert a[0] == b[0] and a[7] == b[7]   # same segments, same deposits
+    assert a[4] < 0.75 * b[4], f"four-wide {a[4]} vs binary {b[4]} node visits"
+
+
+def test_invalid_width_is_rejected():
+    with Context() as ctx:
+        with pytest.raises(Exception):
+            ctx.set_params(bvh_width=3)
