@@ -97,7 +97,7 @@ struct rti_ctx {
   bool batch_mode = false;  // instances came from rti_batch_set (own physics parameters)
   bool have_results = false;
   // scene store
-  dbuf prim_a, prim_b, prim_c, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
+  dbuf prim_a, prim_b, prim_c, prim_n, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
   dbuf s1, s2, cnt, ds, dn, queue, stats;
   // finalisation scratch
   dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj;
@@ -302,6 +302,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->prim_a.ensure(np * 16));
   HIP_TRY(ctx, ctx->prim_b.ensure(np * 16));
   HIP_TRY(ctx, ctx->prim_c.ensure(np * 16));
+  HIP_TRY(ctx, ctx->prim_n.ensure(np * 16));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
   HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
@@ -314,7 +315,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->stats.ensure(64));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
   rti_impl::store_ptrs st{ctx->prim_a.as<float4>(), ctx->prim_b.as<float4>(), ctx->prim_c.as<float4>(),
-                          ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
+                          ctx->prim_n.as<float4>(), ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
                           ctx->nb_ptr.as<uint32_t>(), nullptr, ctx->areas.as<float>()};
   HIP_TRY(ctx, hipError_t(rti_impl::scratch_reserve(ctx->scratch, (uint32_t)maxn)));
   dbuf in_a, in_b;
@@ -513,6 +514,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.prim_a = ctx->prim_a.as<float4>();
   L.prim_b = ctx->prim_b.as<float4>();
   L.prim_c = ctx->prim_c.as<float4>();
+  L.prim_n = ctx->prim_n.as<float4>();
   L.nodes = ctx->nodes.as<bvh_node>();
   L.nodes4 = ctx->nodes4.as<bvh4_node>();
   L.nb_ptr = ctx->nb_ptr.as<uint32_t>();

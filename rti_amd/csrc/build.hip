@@ -209,7 +209,7 @@ __global__ void k_morton(setup_args a, uint32_t* keys, uint32_t* vals) {
 
 // Stored primitive records in Morton order, the permutation, and leaf boxes.
 __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t prim_begin, float4* pa, float4* pb,
-                                float4* pc, uint32_t* perm, float4* leaf_lo, float4* leaf_hi) {
+                                float4* pc, float4* pn, uint32_t* perm, float4* leaf_lo, float4* leaf_hi) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   uint32_t i = order[j];
@@ -223,6 +223,7 @@ __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t pr
     pa[g] = c;
     pb[g] = make_float4(n0, n1, n2, 0.0f);
     pc[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pn[g] = make_float4(n0, n1, n2, 0.0f);
     float nn[3] = {n0, n1, n2};
     float cc[3] = {c.x, c.y, c.z};
 #pragma unroll
@@ -242,6 +243,15 @@ __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t pr
     pa[g] = make_float4(v[0][0], v[0][1], v[0][2], -1.0f);  // w < 0 marks a triangle
     pb[g] = make_float4(v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2], 0.0f);
     pc[g] = make_float4(v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2], 0.0f);
+    {
+      // unit normal from the winding, normalize(e1 x e2), in the plain
+      // (unfused) single-precision arithmetic of this translation unit
+      const float e1x = v[1][0] - v[0][0], e1y = v[1][1] - v[0][1], e1z = v[1][2] - v[0][2];
+      const float e2x = v[2][0] - v[0][0], e2y = v[2][1] - v[0][1], e2z = v[2][2] - v[0][2];
+      const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+      const float l = sqrtf(cx * cx + cy * cy + cz * cz);
+      pn[g] = make_float4(cx / l, cy / l, cz / l, 0.0f);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float l = fminf(v[0][k], fminf(v[1][k], v[2][k]));
@@ -792,7 +802,7 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
     cur ^= 1;
   }
   k_scatter_prims<<<blocks_for(n, T), T, 0, stream>>>(a, s.vals[cur], job.prim_begin, st.prim_a, st.prim_b,
-                                                       st.prim_c, st.perm, s.leaf_lo, s.leaf_hi);
+                                                       st.prim_c, st.prim_n, st.perm, s.leaf_lo, s.leaf_hi);
   e = hipMemsetAsync(s.depth, 0, 4, stream);
   if (e != hipSuccess) return e;
   if (n == 1) {

@@ -31,6 +31,7 @@ struct store_ptrs {
   float4* prim_a;
   float4* prim_b;
   float4* prim_c;
+  float4* prim_n;           // unit normal (discs and triangles)
   uint32_t* perm;           // stored position -> input id (per instance)
   rti_dev::bvh_node* nodes;
   rti_dev::bvh4_node* nodes4;  // four-wide traversal tree, same slot ranges as nodes
@@ -82,6 +83,7 @@ struct trace_launch {
   const float4* prim_a;
   const float4* prim_b;
   const float4* prim_c;
+  const float4* prim_n;
   const rti_dev::bvh_node* nodes;
   const rti_dev::bvh4_node* nodes4;
   const uint32_t* nb_ptr;

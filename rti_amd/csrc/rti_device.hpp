@@ -5,6 +5,7 @@
 //   prim_a[p]  float4   discs: centre xyz + radius;   triangles: v0, w = -1
 //   prim_b[p]  float4   discs: unit normal xyz, 0;    triangles: e1 = v1 - v0
 //   prim_c[p]  float4   triangles: e2 = v2 - v0 (unused for discs)
+//   prim_n[p]  float4   unit normal (discs: as given; triangles: from the winding)
 //   nodes[i]   bvh_node 64 B, both children's boxes in the parent
 //   nodes4[i]  bvh4_node 64 B, four quantised child boxes (the traversal tree)
 //   nb_ptr/nb_col       neighbour CSR over absolute prim ids (discs)

@@ -3,8 +3,10 @@
 For a batch (the same item dicts ``Context.batch_set`` takes) it traces a
 reduced ray budget with each candidate and keeps the fastest, by coordinate
 descent: ``compact_lanes`` (lanes of a wave that must have finished their
-closest-hit query before the wave shades and refills), then the LDS stack
-depth, then the block size. Results are cached in a JSON file keyed by the
+closest-hit query before the wave shades and refills), then the resident
+waves per CU (0 = as many as registers and LDS allow; fewer can suit
+workloads whose working set outgrows the caches), then the LDS stack depth,
+then the block size. Results are cached in a JSON file keyed by the
 batch signature, so a repeated workload is not re-tuned.
 """
 import hashlib
@@ -16,6 +18,7 @@ from ._lib import Context
 COMPACT = (40, 48, 56, 60, 64)
 LDS = (4, 8, 16)
 BLOCK = (128, 256)
+WAVES = (0, 20, 24)
 
 
 def signature(items):
@@ -37,7 +40,7 @@ def _rate(ctx, knobs):
 
 
 def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
-    """Best knobs {compact_lanes, lds_stack, block_size} and their rate."""
+    """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size} and their rate."""
     key = signature(items)
     if cache and os.path.exists(cache):
         with open(cache) as fh:
@@ -45,12 +48,13 @@ def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
         if key in db:
             return db[key]
     small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
-    best = {"compact_lanes": 56, "lds_stack": 8, "block_size": 256}
+    best = {"compact_lanes": 56, "waves_per_cu": 0, "lds_stack": 8, "block_size": 256}
     trials = []
     with Context() as ctx:
         ctx.batch_set(small)
         ctx.prepare()
-        for name, values in (("compact_lanes", COMPACT), ("lds_stack", LDS), ("block_size", BLOCK)):
+        for name, values in (("compact_lanes", COMPACT), ("waves_per_cu", WAVES), ("lds_stack", LDS),
+                             ("block_size", BLOCK)):
             scores = {}
             for v in values:
                 knobs = dict(best, **{name: v})
@@ -73,4 +77,4 @@ def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
 
 
 def knobs(tuned):
-    return {k: tuned[k] for k in ("compact_lanes", "lds_stack", "block_size")}
+    return {k: tuned[k] for k in ("compact_lanes", "waves_per_cu", "lds_stack", "block_size")}
