@@ -17,6 +17,11 @@ Workloads (``--workload``):
   rough  periodic rough surface over 16 holes of aspect 6 (about 4M discs),
          70/30 diffuse/specular, s = 0.02, ``--rays`` (1e9); also reports
          64-bit fixed point vs FP64 atomic accumulation on the same paths.
+  adjoint disc-cloud trench of aspect 8 (192,001 discs at 0.01 spacing),
+         s = 0.05, source exponent n = 5, ``--rays`` (1e8), with d/ds and d/dn
+         accumulated in the same pass; also reports the summed derivatives
+         against central finite differences with common random numbers
+         (ds = 1e-3, dn = 1e-2) over 8 independent seeds, with standard errors.
 
 A step traces the whole workload as one persistent launch replayed from a
 HIP graph, inputs resident in HBM, launch knobs from the autotuner
@@ -81,6 +86,13 @@ def workload(name, rays):
         desc = (f"64 disc-cloud trenches/holes (AR 2-20, {nd} discs total, s log-uniform 1e-3..1, diffuse or "
                 f"specular), {int(r)} rays total proportional to area")
         return cases, desc, 32.0
+    if name == "adjoint":
+        r = 1e8 if rays is None else rays
+        sc = geometry.trench_discs(width=1.0, depth=8.0, flat=1.0, ylen=1.0, spacing=0.01)
+        cases = [dict(scene=sc, n_rays=int(r), sticking=0.05, cos_exponent=5.0, bc_x=0, bc_y=1, src_z=8.1, seed=4)]
+        desc = (f"disc-cloud trench w=1 d=8 (AR 8), 0.01 grid, {sc['xyzr'].shape[0]} discs, reflective x, periodic y, "
+                f"source z=8.1, n=5, s=0.05, {int(r)} rays, adjoint d/ds and d/dn in the same pass")
+        return cases, desc, 32.0
     if name == "rough":
         r = 1e9 if rays is None else rays
         sc = geometry.rough_surface_discs()
@@ -91,21 +103,22 @@ def workload(name, rays):
     raise SystemExit(f"unknown workload {name}")
 
 
-def bytes_per_segment(counters, prim_bytes):
+def bytes_per_segment(counters, prim_bytes, deposit_bytes=48.0):
     """Algorithmic bytes per segment from an instrumented run.
 
-    64 B per BVH node visit (one node record), ``prim_bytes`` per primitive
-    test (triangle 48 B, disc 32 B), 36 B per neighbour test (disc record +
-    column index), 48 B per deposit (read + write of three 8-byte
-    accumulators), 8 B per front hit (neighbour row pointers).
+    64 B per node visit (one node record, binary or four-wide), ``prim_bytes``
+    per primitive test (triangle 48 B, disc 32 B), 36 B per neighbour test
+    (disc record + column index), ``deposit_bytes`` per deposit (read + write
+    of the 8-byte accumulators: three, or five with the adjoint sums), 24 B
+    per front hit (unit normal + neighbour row pointers).
     """
     seg, hits = float(counters[0]), float(counters[1])
     nodes, prims, nbt, deps = (float(x) for x in counters[4:8])
-    total = 64.0 * nodes + prim_bytes * prims + 36.0 * nbt + 48.0 * deps + 8.0 * hits
+    total = 64.0 * nodes + prim_bytes * prims + 36.0 * nbt + deposit_bytes * deps + 24.0 * hits
     return total / seg, {"V_node": nodes / seg, "V_prim": prims / seg, "K_nb": nbt / seg, "D": deps / seg}
 
 
-def cpu_baseline(cases, budget_s=20.0):
+def cpu_baseline(cases, budget_s=20.0, adjoint=False):
     """Oracle (restated reference tracer, OpenMP) on a bounded ray sample:
     a pilot of 2e4 rays per case sizes the sample to about budget_s seconds."""
     import oracle
@@ -120,7 +133,7 @@ def cpu_baseline(cases, budget_s=20.0):
         for c, (box, sz) in zip(cases, boxes):
             r = oracle.trace(c["scene"], n_rays=per_case, box=box, src_z=sz, sticking=c["sticking"],
                              cos_exponent=c.get("cos_exponent", 1.0), diffuse_prob=c.get("diffuse_prob", 1.0),
-                             bc_x=c["bc_x"], bc_y=c["bc_y"], seed=c["seed"], threads=threads)
+                             bc_x=c["bc_x"], bc_y=c["bc_y"], seed=c["seed"], threads=threads, adjoint=adjoint)
             segs += r["segments"]
             secs += r["seconds"]
         return segs, secs
@@ -132,6 +145,40 @@ def cpu_baseline(cases, budget_s=20.0):
     return {"value": segs / secs, "unit": "segments/s", "cores": threads, "kind": "port",
             "sample": f"oracle (restated reference tracer, own BVH, OpenMP) on {per_case} rays for each of "
                       f"{len(cases)} cases of the same geometry: {segs} segments in {secs:.1f} s"}
+
+
+def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
+    """Summed d(raw)/ds and d(raw)/dn of the adjoint pass against central
+    differences of the summed raw flux with common random numbers, repeated
+    over independent seeds: means, standard errors, and the difference of the
+    means in units of its standard error."""
+    from rti_amd import Context
+    base = {k: v for k, v in case.items() if k not in ("scene", "seed")}
+    s0, n0 = base["sticking"], base["cos_exponent"]
+    rows = []
+    with Context(0) as ctx:
+        ctx.set_scene(case["scene"])
+
+        def total(seed, **over):
+            ctx.set_params(**dict(base, **over), seed=seed, **knobs)
+            ctx.run()
+            return float(ctx.flux()["raw"].sum())
+        for k in range(seeds):
+            seed = case["seed"] + 100 + k
+            ctx.set_params(**base, seed=seed, adjoint=1, **knobs)
+            ctx.run()
+            a_s, a_n = (float(x.sum()) for x in ctx.adjoint())
+            f_s = (total(seed, sticking=s0 + ds) - total(seed, sticking=s0 - ds)) / (2 * ds)
+            f_n = (total(seed, cos_exponent=n0 + dn) - total(seed, cos_exponent=n0 - dn)) / (2 * dn)
+            rows.append((a_s, f_s, a_n, f_n))
+    r = np.array(rows)
+    out = {"seeds": seeds, "rays_per_run": base["n_rays"], "steps": {"ds": ds, "dn": dn}}
+    for name, ia, ifd in (("d_ds", 0, 1), ("d_dn", 2, 3)):
+        ma, mf = r[:, ia].mean(), r[:, ifd].mean()
+        sa, sf = r[:, ia].std(ddof=1) / np.sqrt(seeds), r[:, ifd].std(ddof=1) / np.sqrt(seeds)
+        out[name] = {"adjoint": ma, "adjoint_se": sa, "fd": mf, "fd_se": sf, "rel_diff": (ma - mf) / abs(mf),
+                     "z": (ma - mf) / float(np.hypot(sa, sf))}
+    return out
 
 
 def measured_traffic(name, segments_per_launch):
@@ -153,7 +200,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="hole", choices=["hole", "trench", "batch", "rough"])
+    ap.add_argument("--workload", default="hole", choices=["hole", "trench", "batch", "rough", "adjoint"])
     ap.add_argument("--rays", type=float, default=None, help="primary rays (per case for hole, total otherwise)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -172,9 +219,10 @@ def main():
         t = tune.tune(cases, fraction=0.05, log=log)
         knobs = tune.knobs(t)
         log(f"tuned {knobs}")
+    adj = {"adjoint": 1} if args.workload == "adjoint" else {}
     ctx = Context(0)
     ctx.batch_set(cases)
-    ctx.set_params(**knobs)
+    ctx.set_params(**adj, **knobs)
     t0 = time.time()
     ctx.prepare()
     log(f"build {time.time() - t0:.2f} s")
@@ -209,9 +257,13 @@ def main():
         log(f"fixed vs fp64: max rel {rel.max():.3e}, mean rel {rel.mean():.3e}")
         ctx.set_params(**knobs)
 
+    if args.workload == "adjoint":
+        extra["adjoint_vs_fd"] = adjoint_fd_check(cases[0], knobs)
+        log(f"adjoint vs FD: {extra['adjoint_vs_fd']}")
+
     plain = None
     if not args.no_plain:
-        ctx.set_params(kernel=1, **knobs)
+        ctx.set_params(kernel=1, **adj, **knobs)
         ctx.batch_replay(1)
         pst = ctx.batch_stats()
         pc = ctx.counters()
@@ -225,7 +277,7 @@ def main():
     cnt_ctx.batch_set(small)
     cnt_ctx.set_params(kernel=2, **knobs)
     cnt_ctx.batch_run()
-    B, parts = bytes_per_segment(cnt_ctx.counters(), prim_bytes)
+    B, parts = bytes_per_segment(cnt_ctx.counters(), prim_bytes, 80.0 if adj else 48.0)
     cnt_ctx.close()
     achieved = value * B / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -234,7 +286,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cases)
+        cpu = cpu_baseline(cases, adjoint=bool(adj))
         log(f"cpu baseline {cpu['value']:.3e} segments/s on {cpu['cores']} threads")
 
     out = {

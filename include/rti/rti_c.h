@@ -98,6 +98,8 @@ typedef struct rti_params {
                              (1..64; 0 = tuned default)                                */
   int32_t bvh_width;      /* traversal tree: 0 = default (4), 4 = four-wide quantised,
                              2 = binary LBVH                                           */
+  int32_t moments;        /* 1 = also accumulate sums of deposit^3 and deposit^4 (FP64)
+                             for the variance of the variance                          */
 } rti_params;
 
 typedef struct rti_stats {
@@ -160,6 +162,16 @@ int rti_get_areas(rti_ctx* ctx, float* areas);
 /* row_ptr: n_prims + 1 entries; col: up to cap entries (input order ids). */
 int rti_get_neighbors(rti_ctx* ctx, uint32_t* row_ptr, uint32_t* col, uint64_t cap, uint64_t* n_edges);
 int rti_get_stats(rti_ctx* ctx, rti_stats* out);
+/* Per-primitive statistics of the last run, input order, as the reference's
+ * VTP output has them (hit_accumulator::get_relative_error / get_vov,
+ * src/rti/trace/hit_accumulator.hpp:152-249, written by io/vtp_writer.hpp:207-237):
+ * rel_err = sqrt(S2/S1^2 - 1/N), vov = (S4 - 4 S1 S3/N + 8 S2 S1^2/N^2 - 4 S1^4/N^3
+ * - S2^2/N) N^2 / (N S2 - S1^2)^2, with S_k the sums of deposit^k on the
+ * primitive and N all deposits of the run; DBL_MAX where undefined (S1 = 0;
+ * no deposit or zero denominator). vov needs params.moments = 1. Either
+ * output may be NULL. */
+int rti_get_error_stats(rti_ctx* ctx, double* rel_err, double* vov);
+
 /* d(raw)/ds and d(raw)/dn of the last run made with params.adjoint = 1. */
 int rti_get_adjoint(rti_ctx* ctx, double* draw_ds, double* draw_dn);
 /* Counters of the last trace: [0] segments, [1] surface hits, [2] misses,
@@ -192,6 +204,7 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times);
 int rti_batch_get_flux(rti_ctx* ctx, uint32_t i, float* smoothed_norm, double* raw, double* sigma);
 int rti_batch_get_counts(rti_ctx* ctx, uint32_t i, uint64_t* counts);
 int rti_batch_get_stats(rti_ctx* ctx, rti_stats* out);
+int rti_batch_get_error_stats(rti_ctx* ctx, uint32_t i, double* rel_err, double* vov);
 
 #ifdef __cplusplus
 }

@@ -87,7 +87,7 @@ class Params(ctypes.Structure):
         ("diffuse_prob", ctypes.c_float), ("tnear", ctypes.c_float),
         ("roulette", ctypes.c_int32), ("fp64", ctypes.c_int32),
         ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float),
-        ("threads", ctypes.c_int32), ("adjoint", ctypes.c_int32),
+        ("threads", ctypes.c_int32), ("adjoint", ctypes.c_int32), ("moments", ctypes.c_int32),
     ]
 
 
@@ -95,11 +95,13 @@ class Result(ctypes.Structure):
     _fields_ = [
         ("s1", _dp), ("s2", _dp), ("cnt", _u64p), ("dsum_ds", _dp), ("dsum_dn", _dp),
         ("segments", ctypes.c_uint64), ("surface_hits", ctypes.c_uint64),
-        ("misses", ctypes.c_uint64), ("seconds", ctypes.c_double),
+        ("misses", ctypes.c_uint64), ("seconds", ctypes.c_double), ("s3", _dp), ("s4", _dp),
     ]
 
 
 def _declare(L):
+    L.orc_error_stats.argtypes = [_dp, _dp, _dp, _dp, _u64p, ctypes.c_uint64, ctypes.c_uint64, _dp, _dp]
+    L.orc_error_stats.restype = None
     L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
     L.orc_local_intersect.argtypes = [_fp, _fp, _fp, _fp]
     L.orc_local_intersect.restype = ctypes.c_int
@@ -184,10 +186,11 @@ def smooth_normalize(values, areas, row, col):
 
 def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc_y=0, cos_exponent=1.0,
           diffuse_prob=1.0, tnear=1e-4, roulette=True, fp64=False, threads=0, ray_begin=0, adjoint=False,
-          neighbors=None):
+          neighbors=None, moments=False):
     """Run the oracle trace. ``scene`` is a dict from ``rti_amd.geometry``.
 
-    Returns a dict with s1, s2, cnt (per primitive) and the counters.
+    Returns a dict with s1, s2, cnt (per primitive), s3 and s4 with
+    ``moments``, and the counters.
     """
     keep = []
     sc = Scene()
@@ -234,7 +237,11 @@ def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc
     p.src_z = float(src_z)
     p.threads = int(threads)
     p.adjoint = 1 if adjoint else 0
+    p.moments = 1 if moments else 0
     out = {"s1": np.zeros(n), "s2": np.zeros(n), "cnt": np.zeros(n, dtype=np.uint64)}
+    if moments:
+        out["s3"] = np.zeros(n)
+        out["s4"] = np.zeros(n)
     if adjoint:
         out["ds"] = np.zeros(n)
         out["dn"] = np.zeros(n)
@@ -245,9 +252,32 @@ def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc
     if adjoint:
         r.dsum_ds = _ptr(out["ds"], _dp)
         r.dsum_dn = _ptr(out["dn"], _dp)
+    if moments:
+        r.s3 = _ptr(out["s3"], _dp)
+        r.s4 = _ptr(out["s4"], _dp)
     rc = lib().orc_trace(ctypes.byref(sc), ctypes.byref(p), ctypes.byref(r))
     if rc != 0:
         raise RuntimeError(f"orc_trace failed with code {rc}")
     out.update(segments=int(r.segments), surface_hits=int(r.surface_hits), misses=int(r.misses),
                seconds=float(r.seconds))
     return out
+
+
+def error_stats(s1, s2, cnt, s3=None, s4=None, total=None):
+    """Relative error and (with s3, s4) variance of the variance per primitive,
+    from the power sums; total = all deposits of the run (default cnt.sum()).
+    Restates hit_accumulator::get_relative_error / get_vov
+    (src/rti/trace/hit_accumulator.hpp:152-249)."""
+    s1 = np.ascontiguousarray(s1, dtype=np.float64)
+    s2 = np.ascontiguousarray(s2, dtype=np.float64)
+    cnt = np.ascontiguousarray(cnt, dtype=np.uint64)
+    n = s1.shape[0]
+    total = int(cnt.sum()) if total is None else int(total)
+    rel = np.zeros(n)
+    vov = np.zeros(n) if s3 is not None else None
+    z = np.zeros(1)
+    a3 = np.ascontiguousarray(s3, dtype=np.float64) if s3 is not None else z
+    a4 = np.ascontiguousarray(s4, dtype=np.float64) if s4 is not None else z
+    lib().orc_error_stats(_ptr(s1, _dp), _ptr(s2, _dp), _ptr(a3, _dp), _ptr(a4, _dp), _ptr(cnt, _u64p), n, total,
+                          _ptr(rel, _dp), _ptr(vov, _dp) if vov is not None else None)
+    return rel, vov

@@ -28,6 +28,7 @@
 // off so that FP32 results do not depend on whether the host has FMA.
 
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <cfloat>
@@ -370,6 +371,7 @@ struct orc_params {
   float src_z;
   int32_t threads;          // 0 = OpenMP default
   int32_t adjoint;          // 1 = also accumulate d/ds and d/dn estimators
+  int32_t moments;          // 1 = also accumulate sums of deposit^3 and deposit^4
 };
 
 struct orc_result {
@@ -382,6 +384,8 @@ struct orc_result {
   uint64_t surface_hits;    // front-face primitive hits
   uint64_t misses;
   double seconds;           // wall time of the trace region
+  double* s3;               // sum of cubed deposits (moments only; may be null)
+  double* s4;               // sum of deposits^4 (moments only; may be null)
 };
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
@@ -687,7 +691,7 @@ inline v3<R> specular_dir(v3<R> d, v3<R> n) {
 constexpr uint32_t kMaxSegments = 1u << 20;
 
 struct thread_acc {
-  std::vector<double> s1, s2, ds, dn;
+  std::vector<double> s1, s2, s3, s4, ds, dn;
   std::vector<uint64_t> cnt;
   uint64_t segments = 0, hits = 0, misses = 0;
 };
@@ -701,6 +705,10 @@ void trace_range(const scene_view<R>& s, const orc_params& p, thread_acc& acc, u
   auto deposit = [&](uint32_t prim, R drop, double dds, double ddn) {
     acc.s1[prim] += (double)drop;
     acc.s2[prim] += (double)drop * (double)drop;
+    if (p.moments) {  // hit_accumulator::use (src/rti/trace/hit_accumulator.hpp:128-138)
+      acc.s3[prim] += (double)drop * (double)drop * (double)drop;
+      acc.s4[prim] += (double)drop * (double)drop * (double)drop * (double)drop;
+    }
     acc.cnt[prim] += 1;
     if (p.adjoint) {
       acc.ds[prim] += dds;
@@ -799,6 +807,7 @@ int trace_impl(const orc_scene* sc, const orc_params* p, orc_result* res) {
     a.s2.assign(n, 0.0);
     a.cnt.assign(n, 0);
     if (p->adjoint) { a.ds.assign(n, 0.0); a.dn.assign(n, 0.0); }
+    if (p->moments) { a.s3.assign(n, 0.0); a.s4.assign(n, 0.0); }
     const uint64_t chunk = 256;
     uint64_t nchunks = (p->n_rays + chunk - 1) / chunk;
 #pragma omp for schedule(dynamic, 4)
@@ -813,14 +822,17 @@ int trace_impl(const orc_scene* sc, const orc_params* p, orc_result* res) {
   // combine in thread order (tracer.hpp:110-114 combines per-thread copies)
   res->segments = res->surface_hits = res->misses = 0;
   for (uint64_t i = 0; i < n; ++i) {
-    double a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    double a1 = 0, a2 = 0, a3 = 0, a4 = 0, m3 = 0, m4 = 0;
     uint64_t c = 0;
     for (int t = 0; t < nt; ++t) {
       a1 += accs[t].s1[i];
       a2 += accs[t].s2[i];
       c += accs[t].cnt[i];
       if (p->adjoint) { a3 += accs[t].ds[i]; a4 += accs[t].dn[i]; }
+      if (p->moments) { m3 += accs[t].s3[i]; m4 += accs[t].s4[i]; }
     }
+    if (p->moments && res->s3) res->s3[i] = m3;
+    if (p->moments && res->s4) res->s4[i] = m4;
     if (res->s1) res->s1[i] = a1;
     if (res->s2) res->s2[i] = a2;
     if (res->cnt) res->cnt[i] = c;
@@ -857,6 +869,36 @@ int orc_roulette(float* w, float u) {
   if (u < 1.0f - *w / 0.3f) return 0;
   *w = 0.3f;
   return 1;
+}
+
+// Per-primitive relative error and variance of the variance from the power
+// sums, with N = all deposits of the run (hit_accumulator::get_relative_error,
+// src/rti/trace/hit_accumulator.hpp:152-186, and get_vov, :188-249):
+//   rel = sqrt(S2 / S1^2 - 1/N)                     (DBL_MAX where S1 = 0)
+//   vov = (S4 - 4 S1 S3/N + 8 S2 S1^2/N^2 - 4 S1^4/N^3 - S2^2/N) N^2 / (N S2 - S1^2)^2
+//                                                   (DBL_MAX where the count is 0
+//                                                    or the denominator is 0)
+// The powers of N are formed in double (the reference forms N^2 and N^3 in
+// size_t, which wraps for N > 2.6e6 deposits).
+void orc_error_stats(const double* s1, const double* s2, const double* s3, const double* s4, const uint64_t* cnt,
+                     uint64_t n, uint64_t total, double* rel, double* vov) {
+  const double big = std::numeric_limits<double>::max();
+  const double nn = (double)total;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (rel) {
+      const double sq = s1[i] * s1[i];
+      rel[i] = (total == 0 || sq == 0) ? big : std::sqrt(s2[i] / sq - 1.0 / nn);
+    }
+    if (vov) {
+      if (cnt[i] == 0) { vov[i] = big; continue; }
+      const double a1 = s1[i], a2 = s2[i], a3 = s3[i], a4 = s4[i];
+      const double sq = a1 * a1, q4 = sq * sq, a2sq = a2 * a2;
+      const double num = a4 - (4 * a1 * a3 / nn) + (8 * a2 * sq / (nn * nn)) - (4 * q4 / (nn * nn * nn)) - (a2sq / nn);
+      const double r2 = nn * a2 - sq;
+      const double den2 = r2 * r2;
+      vov[i] = den2 == 0 ? big : num / den2 * nn * nn;
+    }
+  }
 }
 
 int orc_trace(const orc_scene* sc, const orc_params* p, orc_result* res) {

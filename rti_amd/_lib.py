@@ -34,7 +34,7 @@ EXPORTS = [
     "rti_get_counts", "rti_get_areas", "rti_get_neighbors", "rti_get_stats", "rti_get_adjoint", "rti_get_counters",
     "rti_get_hierarchy", "rti_get_wide_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
-    "rti_batch_get_stats",
+    "rti_batch_get_stats", "rti_get_error_stats", "rti_batch_get_error_stats",
 ]
 
 
@@ -52,7 +52,7 @@ class Params(ctypes.Structure):
         ("roulette", ctypes.c_int32), ("src_z", ctypes.c_float), ("kernel", ctypes.c_int32),
         ("adjoint", ctypes.c_int32), ("accum", ctypes.c_int32), ("block_size", ctypes.c_int32),
         ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
-        ("bvh_width", ctypes.c_int32),
+        ("bvh_width", ctypes.c_int32), ("moments", ctypes.c_int32),
     ]
 
 
@@ -133,6 +133,8 @@ def _declare(L):
     L.rti_batch_get_flux.argtypes = [vp, c.c_uint32, _fp, _dp, _dp]
     L.rti_batch_get_counts.argtypes = [vp, c.c_uint32, _u64p]
     L.rti_batch_get_stats.argtypes = [vp, c.POINTER(Stats)]
+    L.rti_get_error_stats.argtypes = [vp, _dp, _dp]
+    L.rti_batch_get_error_stats.argtypes = [vp, c.c_uint32, _dp, _dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("rti_default_params", "rti_last_error"):
@@ -283,6 +285,21 @@ class Context:
         self._check(lib().rti_get_counters(self._h, _ptr(c, _u64p)))
         return c
 
+    def error_stats(self, vov=True):
+        """(relative error, variance of the variance or None) per primitive;
+        the variance of the variance needs a run with moments=1."""
+        rel = np.zeros(self.n)
+        v = np.zeros(self.n) if vov else None
+        self._check(lib().rti_get_error_stats(self._h, _ptr(rel, _dp), _ptr(v, _dp) if vov else None))
+        return rel, v
+
+    def batch_error_stats(self, i, vov=True):
+        n = self.batch_n[i]
+        rel = np.zeros(n)
+        v = np.zeros(n) if vov else None
+        self._check(lib().rti_batch_get_error_stats(self._h, i, _ptr(rel, _dp), _ptr(v, _dp) if vov else None))
+        return rel, v
+
     def adjoint(self):
         ds = np.zeros(self.n)
         dn = np.zeros(self.n)
@@ -356,5 +373,6 @@ def run_scene(scene, device=0, **params):
         out["counters"] = ctx.counters()
         if params.get("adjoint"):
             out["ds"], out["dn"] = ctx.adjoint()
+        out["rel_err"], out["vov"] = ctx.error_stats(vov=bool(params.get("moments")))
         return out
 

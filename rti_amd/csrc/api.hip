@@ -77,7 +77,7 @@ struct instance {
   uint64_t n_edges = 0;
   int shift = 32;
   // results (input order)
-  std::vector<double> raw, sigma, dds, ddn;
+  std::vector<double> raw, sigma, dds, ddn, rel, vov;
   std::vector<float> smoothed, areas;
   std::vector<uint64_t> counts;
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -98,13 +98,13 @@ struct rti_ctx {
   bool have_results = false;
   // scene store
   dbuf prim_a, prim_b, prim_c, prim_n, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
-  dbuf s1, s2, cnt, ds, dn, queue, stats;
+  dbuf s1, s2, cnt, ds, dn, s3, s4, queue, stats;
   // finalisation scratch
-  dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj;
+  dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj, out_err;
   rti_impl::build_scratch scratch;
   uint64_t n_total = 0, nodes_total = 0, edges_total = 0, rays_total = 0;
   double build_ms = 0, trace_ms = 0, post_ms = 0;
-  int last_adjoint = 0, last_fp64 = 0, last_count = 0;
+  int last_adjoint = 0, last_fp64 = 0, last_count = 0, last_moments = 0;
   uint64_t run_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -153,6 +153,7 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
   if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
     return fail(ctx, RTI_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
+  if (p.moments != 0 && p.moments != 1) return fail(ctx, RTI_ERR_INVALID_ARG, "moments must be 0 or 1");
   return RTI_OK;
 }
 
@@ -493,6 +494,10 @@ int alloc_accumulators(rti_ctx* ctx, bool adj) {
     HIP_TRY(ctx, ctx->ds.ensure(ctx->n_total * 8));
     HIP_TRY(ctx, ctx->dn.ensure(ctx->n_total * 8));
   }
+  if (ctx->params.moments) {
+    HIP_TRY(ctx, ctx->s3.ensure(ctx->n_total * 8));
+    HIP_TRY(ctx, ctx->s4.ensure(ctx->n_total * 8));
+  }
   return RTI_OK;
 }
 
@@ -506,6 +511,11 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   if (adj) {
     HIP_TRY(ctx, rti_impl::fill_u64(ctx->ds.as<unsigned long long>(), np, 0ull, ctx->stream));
     HIP_TRY(ctx, rti_impl::fill_u64(ctx->dn.as<unsigned long long>(), np, 0ull, ctx->stream));
+  }
+  const bool mom = ctx->params.moments != 0;
+  if (mom) {  // 0.0 has all bits zero
+    HIP_TRY(ctx, rti_impl::fill_u64(ctx->s3.as<unsigned long long>(), np, 0ull, ctx->stream));
+    HIP_TRY(ctx, rti_impl::fill_u64(ctx->s4.as<unsigned long long>(), np, 0ull, ctx->stream));
   }
   HIP_TRY(ctx, rti_impl::fill_u64(ctx->stats.as<unsigned long long>(), 8, 0ull, ctx->stream));
   rti_impl::trace_launch L{};
@@ -524,6 +534,8 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.acc_cnt = ctx->cnt.as<unsigned long long>();
   L.acc_ds = adj ? ctx->ds.as<unsigned long long>() : nullptr;
   L.acc_dn = adj ? ctx->dn.as<unsigned long long>() : nullptr;
+  L.acc_s3 = mom ? ctx->s3.as<double>() : nullptr;
+  L.acc_s4 = mom ? ctx->s4.as<double>() : nullptr;
   L.queue = ctx->queue.as<unsigned long long>();
   L.stats = ctx->stats.as<unsigned long long>();
   L.adjoint = adj ? 1 : 0;
@@ -579,6 +591,7 @@ int trace(rti_ctx* ctx) {
   ctx->last_adjoint = adj;
   ctx->last_fp64 = fp64;
   ctx->last_count = count;
+  ctx->last_moments = ctx->params.moments != 0;
   HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
   ctx->have_results = false;
   return RTI_OK;
@@ -601,6 +614,7 @@ int finalize(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->out_smooth.ensure(maxn * 4));
   HIP_TRY(ctx, ctx->out_f.ensure(maxn * 4));
   if (ctx->last_adjoint) HIP_TRY(ctx, ctx->out_adj.ensure(maxn * 16));
+  HIP_TRY(ctx, ctx->out_err.ensure(maxn * 16));
   for (auto& I : ctx->inst) {
     const uint32_t n = (uint32_t)I.n;
     const double inv = std::ldexp(1.0, -I.shift);
@@ -651,6 +665,26 @@ int finalize(rti_ctx* ctx) {
         return fail(ctx, RTI_ERR_OVERFLOW, "fixed-point range exceeded (%llu deposits on one element)",
                     (unsigned long long)mc);
     }
+    // relative error (always) and variance of the variance (with the moment
+    // sums), N = all deposits of the instance (hit_accumulator.hpp:176)
+    uint64_t total = 0;
+    for (uint64_t c : I.counts) total += c;
+    double* er = ctx->out_err.as<double>();
+    double* ev = er + maxn;
+    const bool mom = ctx->last_moments != 0;
+    HIP_TRY(ctx, rti_impl::error_stats(ctx->s1.as<unsigned long long>(), ctx->s2.as<unsigned long long>(),
+                                       mom ? ctx->s3.as<double>() : nullptr, mom ? ctx->s4.as<double>() : nullptr,
+                                       ctx->cnt.as<unsigned long long>(), ctx->perm.as<uint32_t>(), I.prim_begin, n,
+                                       inv, ctx->last_fp64, total, er, mom ? ev : nullptr, ctx->stream));
+    I.rel.resize(n);
+    HIP_TRY(ctx, hipMemcpyAsync(I.rel.data(), er, n * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    if (mom) {
+      I.vov.resize(n);
+      HIP_TRY(ctx, hipMemcpyAsync(I.vov.data(), ev, n * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+      I.vov.clear();
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
   HIP_TRY(ctx, hipEventRecord(b, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -713,6 +747,15 @@ void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   out->src_z = J.src_z;
   out->wide_stack = I ? I->wide_stack : wstack;
   out->wide_nodes = I ? I->n_wide : nw;
+}
+
+int get_error_impl(rti_ctx* ctx, const instance& I, double* rel_err, double* vov) {
+  if (!ctx->have_results) return fail(ctx, RTI_ERR_STATE, "no results: run first");
+  if (vov && I.vov.empty())
+    return fail(ctx, RTI_ERR_STATE, "variance of the variance needs a run made with params.moments = 1");
+  if (rel_err) std::memcpy(rel_err, I.rel.data(), I.n * 8);
+  if (vov) std::memcpy(vov, I.vov.data(), I.n * 8);
+  return RTI_OK;
 }
 
 }  // namespace
@@ -912,6 +955,18 @@ int rti_get_adjoint(rti_ctx* ctx, double* draw_ds, double* draw_dn) {
   return RTI_OK;
 }
 
+int rti_get_error_stats(rti_ctx* ctx, double* rel_err, double* vov) {
+  if (!ctx) return RTI_ERR_INVALID_ARG;
+  if (ctx->inst.empty()) return fail(ctx, RTI_ERR_NO_GEOMETRY, "no geometry");
+  return get_error_impl(ctx, ctx->inst[0], rel_err, vov);
+}
+
+int rti_batch_get_error_stats(rti_ctx* ctx, uint32_t i, double* rel_err, double* vov) {
+  if (!ctx) return RTI_ERR_INVALID_ARG;
+  if (i >= ctx->inst.size()) return fail(ctx, RTI_ERR_INVALID_ARG, "instance index out of range");
+  return get_error_impl(ctx, ctx->inst[i], rel_err, vov);
+}
+
 int rti_get_counters(rti_ctx* ctx, uint64_t out[8]) {
   if (!ctx || !out) return RTI_ERR_INVALID_ARG;
   std::memcpy(out, ctx->run_stats, 64);
@@ -1033,6 +1088,7 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times) {
   ctx->last_adjoint = adj;
   ctx->last_fp64 = fp64;
   ctx->last_count = 0;
+  ctx->last_moments = ctx->params.moments != 0;
   HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
   ctx->have_results = false;
   return RTI_OK;

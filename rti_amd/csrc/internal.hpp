@@ -93,6 +93,8 @@ struct trace_launch {
   unsigned long long* acc_cnt;
   unsigned long long* acc_ds;       // adjoint, may be null
   unsigned long long* acc_dn;
+  double* acc_s3;                   // sums of deposit^3 and ^4 (FP64 atomics), may be null
+  double* acc_s4;
   unsigned long long* queue;        // [0]: next ray id
   unsigned long long* stats;        // segments, surface hits, misses, truncated
   double fx_scale;                  // 2^shift
@@ -122,6 +124,13 @@ hipError_t finalize_instance(const unsigned long long* s1, const unsigned long l
                              const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                              double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
                              double* sigma_out, unsigned long long* cnt_out, hipStream_t stream);
+// Relative error (rel_out) and, when s3 / s4 are given, the variance of the
+// variance (vov_out) per primitive in input order; total = all deposits of
+// the instance. Either output may be null.
+hipError_t error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3, const double* s4,
+                       const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+                       double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
+                       hipStream_t stream);
 // Signed fixed-point (adjoint) accumulator to double, input order.
 hipError_t finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                            double inv_scale, double* out, hipStream_t stream);

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "internal.hpp"
 
 namespace rti_impl {
@@ -43,6 +45,41 @@ __global__ void k_finalize(const unsigned long long* s1, const unsigned long lon
     sigma_out[i] = var > 0.0 ? sqrt(var) : 0.0;
   }
   if (cnt_out) cnt_out[i] = cnt[g];
+}
+
+// Relative error and variance of the variance per primitive, input order
+// (hit_accumulator::get_relative_error / get_vov,
+// src/rti/trace/hit_accumulator.hpp:152-249), with N = all deposits of the
+// instance:
+//   rel = sqrt(S2 / S1^2 - 1/N)                        (DBL_MAX where S1 = 0)
+//   vov = (S4 - 4 S1 S3/N + 8 S2 S1^2/N^2 - 4 S1^4/N^3 - S2^2/N) N^2 / (N S2 - S1^2)^2
+//                                                      (DBL_MAX where the count or
+//                                                       the denominator is 0)
+// The powers of N are formed in double.
+__global__ void k_error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3,
+                              const double* s4, const unsigned long long* cnt, const uint32_t* perm, uint32_t pb,
+                              uint32_t n, double inv_scale, int fp64, double nn, double* rel, double* vov) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const size_t g = (size_t)pb + j;
+  const uint32_t i = perm[g];
+  const double big = DBL_MAX;
+  const double a1 = acc_value(s1[g], inv_scale, fp64);
+  const double a2 = acc_value(s2[g], inv_scale, fp64);
+  const double sq = a1 * a1;
+  if (rel) rel[i] = (nn == 0.0 || sq == 0.0) ? big : sqrt(a2 / sq - 1.0 / nn);
+  if (vov) {
+    if (cnt[g] == 0) {
+      vov[i] = big;
+      return;
+    }
+    const double a3 = s3[g], a4 = s4[g];
+    const double q4 = sq * sq, a2sq = a2 * a2;
+    const double num = a4 - (4 * a1 * a3 / nn) + (8 * a2 * sq / (nn * nn)) - (4 * q4 / (nn * nn * nn)) - (a2sq / nn);
+    const double r2 = nn * a2 - sq;
+    const double den2 = r2 * r2;
+    vov[i] = den2 == 0.0 ? big : num / den2 * nn * nn;
+  }
 }
 
 __global__ void k_finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t pb, uint32_t n,
@@ -107,6 +144,15 @@ hipError_t finalize_instance(const unsigned long long* s1, const unsigned long l
   k_finalize<<<nblk(n, 256), 256, 0, stream>>>(s1, s2, cnt, perm, prim_begin, n, inv_scale, accum_fp64,
                                                (double)(n_rays ? n_rays : 1), raw_stored, raw_out, sigma_out,
                                                cnt_out);
+  return hipGetLastError();
+}
+
+hipError_t error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3, const double* s4,
+                       const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+                       double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
+                       hipStream_t stream) {
+  k_error_stats<<<nblk(n, 256), 256, 0, stream>>>(s1, s2, s3, s4, cnt, perm, prim_begin, n, inv_scale, accum_fp64,
+                                                  (double)total, rel_out, vov_out);
   return hipGetLastError();
 }
 

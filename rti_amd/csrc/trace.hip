@@ -57,6 +57,8 @@ struct tparams {
   unsigned long long* cnt;
   unsigned long long* ds;
   unsigned long long* dn;
+  double* s3;  // third and fourth power sums (variance of the variance), or null
+  double* s4;
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
@@ -390,6 +392,13 @@ __device__ __forceinline__ void deposit(const tparams& P, uint32_t p, float drop
     atomicAdd(P.s2 + p, (unsigned long long)__double2ll_rn(v * v * (double)scale));
   }
   atomicAdd(P.cnt + p, 1ull);
+  if (P.s3) {
+    // higher moments span too many orders of magnitude for the fixed-point
+    // scale of the first two; they use FP64 atomics (hit_accumulator.hpp:128-138)
+    const double v2 = v * v;
+    unsafeAtomicAdd(P.s3 + p, v2 * v);
+    unsafeAtomicAdd(P.s4 + p, v2 * v * v);
+  }
   if (ADJ) {
     // derivative weights can exceed 1 (by about k s / (1 - s)); they use a
     // scale 2^8 smaller than the primal sums
@@ -725,6 +734,8 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.cnt = L.acc_cnt;
   P.ds = L.acc_ds;
   P.dn = L.acc_dn;
+  P.s3 = L.acc_s3;
+  P.s4 = L.acc_s4;
   P.queue = L.queue;
   P.stats = L.stats;
   P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();

@@ -14,11 +14,11 @@ def small_hole_mesh():
 
 
 def oracle_run(scene, n_rays, sticking, seed=1, bc_x=0, bc_y=0, cos_exponent=1.0, diffuse_prob=1.0, src_z=None,
-               fp64=False, adjoint=False, roulette=True, threads=0, ray_begin=0):
+               fp64=False, adjoint=False, roulette=True, threads=0, ray_begin=0, moments=False):
     box, sz = geometry.scene_box(scene, src_z)
     return oracle.trace(scene, n_rays=n_rays, box=box, src_z=sz, sticking=sticking, seed=seed, bc_x=bc_x, bc_y=bc_y,
                         cos_exponent=cos_exponent, diffuse_prob=diffuse_prob, fp64=fp64, adjoint=adjoint,
-                        roulette=roulette, threads=threads, ray_begin=ray_begin)
+                        roulette=roulette, threads=threads, ray_begin=ray_begin, moments=moments)
 
 
 def zscores(a, sa, b, sb, min_count=None, ca=None, cb=None):

@@ -282,3 +282,52 @@ def test_trace_path_weight_identity():
     fd = (r2["s1"].sum() - r1["s1"].sum()) / (2 * h)
     assert abs(r["ds"].sum() - fd) < 1e-3 * abs(fd)
     assert r["surface_hits"] > 0
+
+
+# ---------------------------------------------------------------------------
+# Relative error and variance of the variance (hit_accumulator.hpp:152-249):
+# the power-sum formulas against the definitions on explicit sample sets.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed,n,p_hit", [(1, 50, 0.5), (2, 400, 0.1), (3, 2000, 0.02), (4, 37, 1.0)])
+def test_error_stats_equal_definitions(seed, n, p_hit):
+    rng = np.random.default_rng(seed)
+    k = 5  # primitives; each history scores on at most one of them
+    who = rng.integers(0, k, n)
+    hit = rng.random(n) < p_hit
+    val = rng.random(n) * 0.9 + 0.05
+    x = np.zeros((k, n))
+    x[who[hit], np.nonzero(hit)[0]] = val[hit]
+    s = [x.sum(1), (x ** 2).sum(1), (x ** 3).sum(1), (x ** 4).sum(1)]
+    cnt = (x > 0).sum(1).astype(np.uint64)
+    # N = all deposits of the run; here every nonzero score is one deposit
+    total = int(cnt.sum())
+    rel, vov = oracle.error_stats(s[0], s[1], cnt, s[2], s[3], total=total)
+    for i in range(k):
+        if cnt[i] == 0:
+            assert rel[i] == np.finfo(np.float64).max and vov[i] == np.finfo(np.float64).max
+            continue
+        # the same statistics over N = total samples (zeros for other deposits)
+        xi = np.zeros(total)
+        xi[: int(cnt[i])] = x[i][x[i] > 0]
+        d = xi - xi.mean()
+        m2 = (d ** 2).sum()
+        # relative error of the mean with sqrt(N - 1) ~ sqrt(N)
+        assert rel[i] == pytest.approx(np.sqrt(m2) / xi.sum(), rel=1e-10)
+        if m2 > 0:
+            assert vov[i] == pytest.approx((d ** 4).sum() / m2 ** 2 - 1.0 / total, rel=1e-8, abs=1e-12)
+
+
+def test_error_stats_undefined_cases():
+    big = np.finfo(np.float64).max
+    s1 = np.array([0.0, 2.0, 3.0])
+    s2 = np.array([0.0, 2.0, 3.0])
+    cnt = np.array([0, 2, 3], dtype=np.uint64)
+    # element 0: S1 = 0 and no deposit; element 2 holds every deposit of a run of
+    # equal values, so N S2 - S1^2 = 0
+    rel, vov = oracle.error_stats(s1, s2, cnt, s2, s2, total=3)
+    assert rel[0] == big and vov[0] == big
+    assert vov[2] == big
+    assert rel[2] == pytest.approx(0.0, abs=1e-12)
+    # no deposits at all: every relative error is undefined
+    rel0, _ = oracle.error_stats(np.zeros(2), np.zeros(2), np.zeros(2, dtype=np.uint64), total=0)
+    assert np.all(rel0 == big)
