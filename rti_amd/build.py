@@ -55,5 +55,22 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+CLI_SRC = os.path.join(HERE, "cli", "rti_cli.cpp")
+CLI = os.path.join(HERE, "cli", "rti_cli")
+
+
+def build_cli(verbose=False):
+    """The command-line tracer (host C++ over the C ABI, VTK XML I/O, zlib)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + inc, CLI_SRC, "-o", CLI + ".tmp", "-L" + HERE, "-lrti_hip",
+           "-lz", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cli(verbose="-v" in sys.argv))

@@ -233,6 +233,9 @@ void geometry_bounds(instance& I) {
 // centres (boundary_x_y.hpp:194-207 via device.hpp:93); the source sits at the
 // padded top (device.hpp:257-267) unless params.src_z overrides it, in which
 // case the lateral faces are extended up to the source.
+// The boundary faces reach up to the source plane, as in the reference,
+// where the source sits on the top of the bounding box (device.hpp:89-93,
+// 257-267; main/dev.cpp, which pads the box top once instead of twice).
 void derive_box(instance& I) {
   float sz = std::isnan(I.params.src_z) ? I.ztop : I.params.src_z;
   I.src_z = sz;
@@ -241,7 +244,7 @@ void derive_box(instance& I) {
   I.box[2] = I.geo_lo[1];
   I.box[3] = I.geo_hi[1];
   I.box[4] = I.geo_lo[2];
-  I.box[5] = std::max(I.ztop, sz);
+  I.box[5] = sz;
 }
 
 void centroid_bounds(const instance& I, float cmin[3], float cmax[3]) {

@@ -232,7 +232,8 @@ def scene_box(scene, src_z=None):
     centre + 2 * max radius (src/rti/geo/point_cloud_disc_geometry.hpp:83-87
     and src/rti/device.hpp:88-90, 224-237); the source plane sits at that top
     (device.hpp:257-267). Triangles: vertex extent, source 1% of the lateral
-    extent above the top. An explicit ``src_z`` raises the box top to it.
+    extent above the top. An explicit ``src_z`` moves the source plane and
+    with it the top of the boundary faces.
     Returns (box[6] = xmin, xmax, ymin, ymax, zmin, zmax, src_z) in float32.
     """
     f = np.float32
@@ -248,7 +249,7 @@ def scene_box(scene, src_z=None):
         hi = v.max(0)
         ztop = f(hi[2] + f(0.01) * f(max(hi[0] - lo[0], hi[1] - lo[1])))
     sz = ztop if src_z is None else f(src_z)
-    box = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], max(ztop, sz)], dtype=np.float32)
+    box = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], sz], dtype=np.float32)
     return box, f(sz)
 
 
