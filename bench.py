@@ -181,6 +181,19 @@ def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
     return out
 
 
+def measured_valu_busy(name):
+    """Fraction of cycles the SIMDs issued VALU instructions in the trace
+    kernel, from the committed counter summary (SQ_ACTIVE_INST_VALU), or None."""
+    path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if "/ busy cycles =" in line:
+                return float(line.split("=")[-1])
+    return None
+
+
 def measured_traffic(name, segments_per_launch):
     """HBM-side traffic per launch from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE of the trace kernel, each scaled by what the
@@ -216,7 +229,7 @@ def main():
     log(f"{args.workload}: {desc}")
     knobs = {}
     if not args.no_tune:
-        t = tune.tune(cases, fraction=0.05, log=log)
+        t = tune.tune(cases, log=log)
         knobs = tune.knobs(t)
         log(f"tuned {knobs}")
     adj = {"adjoint": 1} if args.workload == "adjoint" else {}
@@ -282,7 +295,10 @@ def main():
     achieved = value * B / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, segments),
-                "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()}}
+                "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
+                "valu_busy": measured_valu_busy(args.workload),
+                "note": "the trace kernel is limited by VALU issue (valu_busy, from the committed SQ counter "
+                        "summary), not by HBM bandwidth; frac is algorithmic bytes over the HBM peak"}
 
     cpu = None
     if not args.no_cpu:

@@ -34,12 +34,21 @@ def signature(items):
 def _rate(ctx, knobs):
     ctx.set_params(**knobs)
     ctx.batch_replay(1)          # capture + warm
-    ctx.batch_replay(2)
+    ctx.batch_replay(1)
     st = ctx.batch_stats()
     return float(ctx.counters()[0]) / (st["trace_ms"] / 1e3)
 
 
-def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
+def default_fraction(items, sample_rays=5e7, full_up_to=1e8):
+    """Fraction of the rays the candidates are timed on: all of them up to
+    ``full_up_to`` rays (a small sample of a batch is dominated by its single
+    longest path, which does not shrink with the sample), else about
+    ``sample_rays``."""
+    total = float(sum(it["n_rays"] for it in items))
+    return 1.0 if total <= full_up_to else sample_rays / total
+
+
+def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size} and their rate."""
     key = signature(items)
     if cache and os.path.exists(cache):
@@ -47,6 +56,8 @@ def tune(items, fraction=0.05, min_rays=20000, cache=None, log=None):
             db = json.load(fh)
         if key in db:
             return db[key]
+    if fraction is None:
+        fraction = default_fraction(items)
     small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
     best = {"compact_lanes": 56, "waves_per_cu": 0, "lds_stack": 8, "block_size": 256}
     trials = []
