@@ -205,6 +205,8 @@ int rti_batch_get_flux(rti_ctx* ctx, uint32_t i, float* smoothed_norm, double* r
 int rti_batch_get_counts(rti_ctx* ctx, uint32_t i, uint64_t* counts);
 int rti_batch_get_stats(rti_ctx* ctx, rti_stats* out);
 int rti_batch_get_error_stats(rti_ctx* ctx, uint32_t i, double* rel_err, double* vov);
+/* d(raw)/ds and d(raw)/dn of instance i after a batch run with params.adjoint = 1. */
+int rti_batch_get_adjoint(rti_ctx* ctx, uint32_t i, double* draw_ds, double* draw_dn);
 
 #ifdef __cplusplus
 }

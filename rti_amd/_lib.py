@@ -34,7 +34,7 @@ EXPORTS = [
     "rti_get_counts", "rti_get_areas", "rti_get_neighbors", "rti_get_stats", "rti_get_adjoint", "rti_get_counters",
     "rti_get_hierarchy", "rti_get_wide_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
-    "rti_batch_get_stats", "rti_get_error_stats", "rti_batch_get_error_stats",
+    "rti_batch_get_stats", "rti_get_error_stats", "rti_batch_get_error_stats", "rti_batch_get_adjoint",
 ]
 
 
@@ -135,6 +135,7 @@ def _declare(L):
     L.rti_batch_get_stats.argtypes = [vp, c.POINTER(Stats)]
     L.rti_get_error_stats.argtypes = [vp, _dp, _dp]
     L.rti_batch_get_error_stats.argtypes = [vp, c.c_uint32, _dp, _dp]
+    L.rti_batch_get_adjoint.argtypes = [vp, c.c_uint32, _dp, _dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("rti_default_params", "rti_last_error"):
@@ -299,6 +300,13 @@ class Context:
         v = np.zeros(n) if vov else None
         self._check(lib().rti_batch_get_error_stats(self._h, i, _ptr(rel, _dp), _ptr(v, _dp) if vov else None))
         return rel, v
+
+    def batch_adjoint(self, i):
+        n = self.batch_n[i]
+        ds = np.zeros(n)
+        dn = np.zeros(n)
+        self._check(lib().rti_batch_get_adjoint(self._h, i, _ptr(ds, _dp), _ptr(dn, _dp)))
+        return ds, dn
 
     def adjoint(self):
         ds = np.zeros(self.n)

@@ -97,8 +97,8 @@ struct rti_ctx {
   bool batch_mode = false;  // instances came from rti_batch_set (own physics parameters)
   bool have_results = false;
   // scene store
-  dbuf prim_a, prim_b, prim_c, prim_n, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
-  dbuf s1, s2, cnt, ds, dn, s3, s4, queue, stats;
+  dbuf prims, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
+  dbuf acc, adj, mom, queue, stats;  // accumulator records (rti_device.hpp)
   // finalisation scratch
   dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj, out_err;
   rti_impl::build_scratch scratch;
@@ -109,6 +109,7 @@ struct rti_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
+  std::vector<inst_desc> desc_host;  // source of the descriptor upload; lives as long as the context
 };
 
 namespace {
@@ -303,23 +304,17 @@ int prepare(rti_ctx* ctx) {
   if (np >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives in total");
   ctx->n_total = np;
   ctx->nodes_total = nn;
-  HIP_TRY(ctx, ctx->prim_a.ensure(np * 16));
-  HIP_TRY(ctx, ctx->prim_b.ensure(np * 16));
-  HIP_TRY(ctx, ctx->prim_c.ensure(np * 16));
-  HIP_TRY(ctx, ctx->prim_n.ensure(np * 16));
+  HIP_TRY(ctx, ctx->prims.ensure(np * 16 * rti_dev::kPrimStride));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
   HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
   HIP_TRY(ctx, ctx->nb_ptr.ensure((np + 1) * 4));
   HIP_TRY(ctx, ctx->areas.ensure(np * 4));
-  HIP_TRY(ctx, ctx->s1.ensure(np * 8));
-  HIP_TRY(ctx, ctx->s2.ensure(np * 8));
-  HIP_TRY(ctx, ctx->cnt.ensure(np * 8));
+  HIP_TRY(ctx, ctx->acc.ensure(np * 8 * rti_dev::kAccStride));
   HIP_TRY(ctx, ctx->queue.ensure(64));
   HIP_TRY(ctx, ctx->stats.ensure(64));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
-  rti_impl::store_ptrs st{ctx->prim_a.as<float4>(), ctx->prim_b.as<float4>(), ctx->prim_c.as<float4>(),
-                          ctx->prim_n.as<float4>(), ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
+  rti_impl::store_ptrs st{ctx->prims.as<float4>(), ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
                           ctx->nb_ptr.as<uint32_t>(), nullptr, ctx->areas.as<float>()};
   HIP_TRY(ctx, hipError_t(rti_impl::scratch_reserve(ctx->scratch, (uint32_t)maxn)));
   dbuf in_a, in_b;
@@ -421,7 +416,8 @@ int upload_descriptors(rti_ctx* ctx) {
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
     return path_cost(ctx->inst[a].params) > path_cost(ctx->inst[b].params);
   });
-  std::vector<inst_desc> D(n);
+  std::vector<inst_desc>& D = ctx->desc_host;  // the asynchronous copy reads it after this returns
+  D.assign(n, inst_desc{});
   uint64_t rb = 0;
   for (size_t j = 0; j < n; ++j) {
     const size_t k = order[j];
@@ -493,14 +489,8 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
 }
 
 int alloc_accumulators(rti_ctx* ctx, bool adj) {
-  if (adj) {
-    HIP_TRY(ctx, ctx->ds.ensure(ctx->n_total * 8));
-    HIP_TRY(ctx, ctx->dn.ensure(ctx->n_total * 8));
-  }
-  if (ctx->params.moments) {
-    HIP_TRY(ctx, ctx->s3.ensure(ctx->n_total * 8));
-    HIP_TRY(ctx, ctx->s4.ensure(ctx->n_total * 8));
-  }
+  if (adj) HIP_TRY(ctx, ctx->adj.ensure(ctx->n_total * 16));
+  if (ctx->params.moments) HIP_TRY(ctx, ctx->mom.ensure(ctx->n_total * 16));
   return RTI_OK;
 }
 
@@ -508,37 +498,23 @@ int alloc_accumulators(rti_ctx* ctx, bool adj) {
 // `only` selects one instance for the plain per-geometry path (-1: all).
 int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool count, int kernel, long only) {
   const uint64_t np = ctx->n_total;
-  HIP_TRY(ctx, rti_impl::fill_u64(ctx->s1.as<unsigned long long>(), np, 0ull, ctx->stream));
-  HIP_TRY(ctx, rti_impl::fill_u64(ctx->s2.as<unsigned long long>(), np, 0ull, ctx->stream));
-  HIP_TRY(ctx, rti_impl::fill_u64(ctx->cnt.as<unsigned long long>(), np, 0ull, ctx->stream));
-  if (adj) {
-    HIP_TRY(ctx, rti_impl::fill_u64(ctx->ds.as<unsigned long long>(), np, 0ull, ctx->stream));
-    HIP_TRY(ctx, rti_impl::fill_u64(ctx->dn.as<unsigned long long>(), np, 0ull, ctx->stream));
-  }
+  HIP_TRY(ctx, rti_impl::fill_u64(ctx->acc.as<unsigned long long>(), np * rti_dev::kAccStride, 0ull, ctx->stream));
+  if (adj) HIP_TRY(ctx, rti_impl::fill_u64(ctx->adj.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
   const bool mom = ctx->params.moments != 0;
-  if (mom) {  // 0.0 has all bits zero
-    HIP_TRY(ctx, rti_impl::fill_u64(ctx->s3.as<unsigned long long>(), np, 0ull, ctx->stream));
-    HIP_TRY(ctx, rti_impl::fill_u64(ctx->s4.as<unsigned long long>(), np, 0ull, ctx->stream));
-  }
+  if (mom)  // 0.0 has all bits zero
+    HIP_TRY(ctx, rti_impl::fill_u64(ctx->mom.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
   HIP_TRY(ctx, rti_impl::fill_u64(ctx->stats.as<unsigned long long>(), 8, 0ull, ctx->stream));
   rti_impl::trace_launch L{};
   L.desc = ctx->desc.as<inst_desc>();
   L.n_inst = (uint32_t)ctx->inst.size();
-  L.prim_a = ctx->prim_a.as<float4>();
-  L.prim_b = ctx->prim_b.as<float4>();
-  L.prim_c = ctx->prim_c.as<float4>();
-  L.prim_n = ctx->prim_n.as<float4>();
+  L.prims = ctx->prims.as<float4>();
   L.nodes = ctx->nodes.as<bvh_node>();
   L.nodes4 = ctx->nodes4.as<bvh4_node>();
   L.nb_ptr = ctx->nb_ptr.as<uint32_t>();
   L.nb_col = ctx->nb_col.as<uint32_t>();
-  L.acc_s1 = ctx->s1.as<unsigned long long>();
-  L.acc_s2 = ctx->s2.as<unsigned long long>();
-  L.acc_cnt = ctx->cnt.as<unsigned long long>();
-  L.acc_ds = adj ? ctx->ds.as<unsigned long long>() : nullptr;
-  L.acc_dn = adj ? ctx->dn.as<unsigned long long>() : nullptr;
-  L.acc_s3 = mom ? ctx->s3.as<double>() : nullptr;
-  L.acc_s4 = mom ? ctx->s4.as<double>() : nullptr;
+  L.acc = ctx->acc.as<unsigned long long>();
+  L.adj = adj ? ctx->adj.as<unsigned long long>() : nullptr;
+  L.mom = mom ? ctx->mom.as<double>() : nullptr;
   L.queue = ctx->queue.as<unsigned long long>();
   L.stats = ctx->stats.as<unsigned long long>();
   L.adjoint = adj ? 1 : 0;
@@ -622,8 +598,7 @@ int finalize(rti_ctx* ctx) {
     const uint32_t n = (uint32_t)I.n;
     const double inv = std::ldexp(1.0, -I.shift);
     HIP_TRY(ctx, rti_impl::finalize_instance(
-                     ctx->s1.as<unsigned long long>(), ctx->s2.as<unsigned long long>(),
-                     ctx->cnt.as<unsigned long long>(), ctx->perm.as<uint32_t>(), I.prim_begin, n, inv,
+                     ctx->acc.as<unsigned long long>(), ctx->perm.as<uint32_t>(), I.prim_begin, n, inv,
                      ctx->last_fp64, I.params.n_rays, ctx->raw_st.as<double>(), ctx->out_raw.as<double>(),
                      ctx->out_sigma.as<double>(), ctx->out_cnt.as<unsigned long long>(), ctx->stream));
     HIP_TRY(ctx, rti_impl::smooth_normalize(ctx->raw_st.as<double>(), ctx->areas.as<float>(),
@@ -646,9 +621,9 @@ int finalize(rti_ctx* ctx) {
       const double inva = std::ldexp(1.0, -(I.shift - 8));
       double* o1 = ctx->out_adj.as<double>();
       double* o2 = o1 + maxn;
-      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->ds.as<unsigned long long>(), ctx->perm.as<uint32_t>(),
+      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 0, ctx->perm.as<uint32_t>(),
                                              I.prim_begin, n, inva, o1, ctx->stream));
-      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->dn.as<unsigned long long>(), ctx->perm.as<uint32_t>(),
+      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 1, ctx->perm.as<uint32_t>(),
                                              I.prim_begin, n, inva, o2, ctx->stream));
       I.dds.resize(n);
       I.ddn.resize(n);
@@ -675,10 +650,9 @@ int finalize(rti_ctx* ctx) {
     double* er = ctx->out_err.as<double>();
     double* ev = er + maxn;
     const bool mom = ctx->last_moments != 0;
-    HIP_TRY(ctx, rti_impl::error_stats(ctx->s1.as<unsigned long long>(), ctx->s2.as<unsigned long long>(),
-                                       mom ? ctx->s3.as<double>() : nullptr, mom ? ctx->s4.as<double>() : nullptr,
-                                       ctx->cnt.as<unsigned long long>(), ctx->perm.as<uint32_t>(), I.prim_begin, n,
-                                       inv, ctx->last_fp64, total, er, mom ? ev : nullptr, ctx->stream));
+    HIP_TRY(ctx, rti_impl::error_stats(ctx->acc.as<unsigned long long>(), mom ? ctx->mom.as<double>() : nullptr,
+                                       ctx->perm.as<uint32_t>(), I.prim_begin, n, inv, ctx->last_fp64, total, er,
+                                       mom ? ev : nullptr, ctx->stream));
     I.rel.resize(n);
     HIP_TRY(ctx, hipMemcpyAsync(I.rel.data(), er, n * 8ull, hipMemcpyDeviceToHost, ctx->stream));
     if (mom) {
@@ -970,6 +944,17 @@ int rti_batch_get_error_stats(rti_ctx* ctx, uint32_t i, double* rel_err, double*
   return get_error_impl(ctx, ctx->inst[i], rel_err, vov);
 }
 
+int rti_batch_get_adjoint(rti_ctx* ctx, uint32_t i, double* draw_ds, double* draw_dn) {
+  if (!ctx) return RTI_ERR_INVALID_ARG;
+  if (i >= ctx->inst.size()) return fail(ctx, RTI_ERR_INVALID_ARG, "instance index out of range");
+  if (!ctx->have_results) return fail(ctx, RTI_ERR_STATE, "no results: call rti_batch_run first");
+  const instance& I = ctx->inst[i];
+  if (I.dds.empty()) return fail(ctx, RTI_ERR_STATE, "last run was not made with params.adjoint = 1");
+  if (draw_ds) std::memcpy(draw_ds, I.dds.data(), I.n * 8);
+  if (draw_dn) std::memcpy(draw_dn, I.ddn.data(), I.n * 8);
+  return RTI_OK;
+}
+
 int rti_get_counters(rti_ctx* ctx, uint64_t out[8]) {
   if (!ctx || !out) return RTI_ERR_INVALID_ARG;
   std::memcpy(out, ctx->run_stats, 64);
@@ -992,8 +977,9 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
                            hipMemcpyDeviceToHost));
   }
   if (perm) HIP_TRY(ctx, hipMemcpy(perm, ctx->perm.as<uint32_t>() + I.prim_begin, I.n * 4, hipMemcpyDeviceToHost));
-  if (prim_a)
-    HIP_TRY(ctx, hipMemcpy(prim_a, ctx->prim_a.as<float4>() + I.prim_begin, I.n * 16, hipMemcpyDeviceToHost));
+  if (prim_a)  // the first float4 of each primitive record
+    HIP_TRY(ctx, hipMemcpy2D(prim_a, 16, ctx->prims.as<float4>() + (size_t)rti_dev::kPrimStride * I.prim_begin,
+                             16 * rti_dev::kPrimStride, 16, I.n, hipMemcpyDeviceToHost));
   return RTI_OK;
 }
 

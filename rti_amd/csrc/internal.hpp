@@ -28,10 +28,7 @@ struct build_job {
 
 // Scene store (absolute indexing; see rti_device.hpp).
 struct store_ptrs {
-  float4* prim_a;
-  float4* prim_b;
-  float4* prim_c;
-  float4* prim_n;           // unit normal (discs and triangles)
+  float4* prims;            // kPrimStride float4 per primitive (see rti_device.hpp)
   uint32_t* perm;           // stored position -> input id (per instance)
   rti_dev::bvh_node* nodes;
   rti_dev::bvh4_node* nodes4;  // four-wide traversal tree, same slot ranges as nodes
@@ -80,21 +77,14 @@ struct trace_launch {
   const rti_dev::inst_desc* desc;   // n_inst descriptors
   uint32_t n_inst;
   uint64_t ray_lo, ray_hi;          // global ray ids [ray_lo, ray_hi) traced by this launch
-  const float4* prim_a;
-  const float4* prim_b;
-  const float4* prim_c;
-  const float4* prim_n;
+  const float4* prims;
   const rti_dev::bvh_node* nodes;
   const rti_dev::bvh4_node* nodes4;
   const uint32_t* nb_ptr;
   const uint32_t* nb_col;
-  unsigned long long* acc_s1;       // fixed point (or double bits in FP64 mode)
-  unsigned long long* acc_s2;
-  unsigned long long* acc_cnt;
-  unsigned long long* acc_ds;       // adjoint, may be null
-  unsigned long long* acc_dn;
-  double* acc_s3;                   // sums of deposit^3 and ^4 (FP64 atomics), may be null
-  double* acc_s4;
+  unsigned long long* acc;          // kAccStride per primitive: sum (fixed point or double bits), sum of squares, count
+  unsigned long long* adj;          // 2 per primitive: d/ds, d/dn (signed fixed point); may be null
+  double* mom;                      // 2 per primitive: sums of deposit^3, ^4 (FP64 atomics); may be null
   unsigned long long* queue;        // [0]: next ray id
   unsigned long long* stats;        // segments, surface hits, misses, truncated
   double fx_scale;                  // 2^shift
@@ -120,19 +110,18 @@ int default_compact_lanes();
 hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);
 // Converts one instance's accumulators: raw_stored[j] (local stored order)
 // and, in input order via perm, raw_out / sigma_out / cnt_out (each may be null).
-hipError_t finalize_instance(const unsigned long long* s1, const unsigned long long* s2,
-                             const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+hipError_t finalize_instance(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                              double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
                              double* sigma_out, unsigned long long* cnt_out, hipStream_t stream);
 // Relative error (rel_out) and, when s3 / s4 are given, the variance of the
 // variance (vov_out) per primitive in input order; total = all deposits of
 // the instance. Either output may be null.
-hipError_t error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3, const double* s4,
-                       const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                       double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
+hipError_t error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t prim_begin,
+                       uint32_t n, double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
                        hipStream_t stream);
-// Signed fixed-point (adjoint) accumulator to double, input order.
-hipError_t finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+// Signed fixed-point (adjoint) accumulator component k of adj[2p + k] to
+// double, input order.
+hipError_t finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                            double inv_scale, double* out, hipStream_t stream);
 // Smoothed, max-normalised estimate (reference device.hpp:153-187), written
 // in input order. raw_stored / areas are in local stored order; the CSR uses

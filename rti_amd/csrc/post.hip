@@ -27,24 +27,25 @@ __device__ __forceinline__ double acc_value(unsigned long long x, double inv_sca
   return fp64 ? __longlong_as_double((long long)x) : (double)(long long)x * inv_scale;
 }
 
-__global__ void k_finalize(const unsigned long long* s1, const unsigned long long* s2, const unsigned long long* cnt,
-                           const uint32_t* perm, uint32_t pb, uint32_t n, double inv_scale, int fp64, double nrays,
-                           double* raw_stored, double* raw_out, double* sigma_out, unsigned long long* cnt_out) {
+__global__ void k_finalize(const unsigned long long* acc, const uint32_t* perm, uint32_t pb, uint32_t n,
+                           double inv_scale, int fp64, double nrays, double* raw_stored, double* raw_out,
+                           double* sigma_out, unsigned long long* cnt_out) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   size_t g = (size_t)pb + j;
-  double v = acc_value(s1[g], inv_scale, fp64);
+  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * g;
+  double v = acc_value(a[0], inv_scale, fp64);
   raw_stored[j] = v;
   uint32_t i = perm[g];
   if (raw_out) raw_out[i] = v;
   if (sigma_out) {
     // standard deviation of the sum over nrays histories, from the
     // per-deposit second moment: var = S2 - S1^2 / N
-    double q = acc_value(s2[g], inv_scale, fp64);
+    double q = acc_value(a[1], inv_scale, fp64);
     double var = q - v * v / nrays;
     sigma_out[i] = var > 0.0 ? sqrt(var) : 0.0;
   }
-  if (cnt_out) cnt_out[i] = cnt[g];
+  if (cnt_out) cnt_out[i] = a[2];
 }
 
 // Relative error and variance of the variance per primitive, input order
@@ -56,24 +57,24 @@ __global__ void k_finalize(const unsigned long long* s1, const unsigned long lon
 //                                                      (DBL_MAX where the count or
 //                                                       the denominator is 0)
 // The powers of N are formed in double.
-__global__ void k_error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3,
-                              const double* s4, const unsigned long long* cnt, const uint32_t* perm, uint32_t pb,
+__global__ void k_error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t pb,
                               uint32_t n, double inv_scale, int fp64, double nn, double* rel, double* vov) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const size_t g = (size_t)pb + j;
   const uint32_t i = perm[g];
   const double big = DBL_MAX;
-  const double a1 = acc_value(s1[g], inv_scale, fp64);
-  const double a2 = acc_value(s2[g], inv_scale, fp64);
+  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * g;
+  const double a1 = acc_value(a[0], inv_scale, fp64);
+  const double a2 = acc_value(a[1], inv_scale, fp64);
   const double sq = a1 * a1;
   if (rel) rel[i] = (nn == 0.0 || sq == 0.0) ? big : sqrt(a2 / sq - 1.0 / nn);
   if (vov) {
-    if (cnt[g] == 0) {
+    if (a[2] == 0) {
       vov[i] = big;
       return;
     }
-    const double a3 = s3[g], a4 = s4[g];
+    const double a3 = mom[2 * g], a4 = mom[2 * g + 1];
     const double q4 = sq * sq, a2sq = a2 * a2;
     const double num = a4 - (4 * a1 * a3 / nn) + (8 * a2 * sq / (nn * nn)) - (4 * q4 / (nn * nn * nn)) - (a2sq / nn);
     const double r2 = nn * a2 - sq;
@@ -82,12 +83,12 @@ __global__ void k_error_stats(const unsigned long long* s1, const unsigned long 
   }
 }
 
-__global__ void k_finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t pb, uint32_t n,
+__global__ void k_finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t pb, uint32_t n,
                                   double inv_scale, double* out) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   size_t g = (size_t)pb + j;
-  out[perm[g]] = (double)(long long)acc[g] * inv_scale;
+  out[perm[g]] = (double)(long long)adj[2 * g + k] * inv_scale;
 }
 
 __global__ void k_smooth(const double* raw, const float* areas, const uint32_t* nb_ptr, const uint32_t* nb_col,
@@ -137,28 +138,26 @@ hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hip
   return hipGetLastError();
 }
 
-hipError_t finalize_instance(const unsigned long long* s1, const unsigned long long* s2,
-                             const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+hipError_t finalize_instance(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                              double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
                              double* sigma_out, unsigned long long* cnt_out, hipStream_t stream) {
-  k_finalize<<<nblk(n, 256), 256, 0, stream>>>(s1, s2, cnt, perm, prim_begin, n, inv_scale, accum_fp64,
+  k_finalize<<<nblk(n, 256), 256, 0, stream>>>(acc, perm, prim_begin, n, inv_scale, accum_fp64,
                                                (double)(n_rays ? n_rays : 1), raw_stored, raw_out, sigma_out,
                                                cnt_out);
   return hipGetLastError();
 }
 
-hipError_t error_stats(const unsigned long long* s1, const unsigned long long* s2, const double* s3, const double* s4,
-                       const unsigned long long* cnt, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                       double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
+hipError_t error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t prim_begin,
+                       uint32_t n, double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
                        hipStream_t stream) {
-  k_error_stats<<<nblk(n, 256), 256, 0, stream>>>(s1, s2, s3, s4, cnt, perm, prim_begin, n, inv_scale, accum_fp64,
-                                                  (double)total, rel_out, vov_out);
+  k_error_stats<<<nblk(n, 256), 256, 0, stream>>>(acc, mom, perm, prim_begin, n, inv_scale, accum_fp64, (double)total,
+                                                  rel_out, vov_out);
   return hipGetLastError();
 }
 
-hipError_t finalize_signed(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
+hipError_t finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
                            double inv_scale, double* out, hipStream_t stream) {
-  k_finalize_signed<<<nblk(n, 256), 256, 0, stream>>>(acc, perm, prim_begin, n, inv_scale, out);
+  k_finalize_signed<<<nblk(n, 256), 256, 0, stream>>>(adj, k, perm, prim_begin, n, inv_scale, out);
   return hipGetLastError();
 }
 

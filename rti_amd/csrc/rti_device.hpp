@@ -2,14 +2,21 @@
 //
 // Layout in HBM (one "scene store" holds one geometry or a whole batch; all
 // arrays are concatenated over the instances and indexed by absolute ids):
-//   prim_a[p]  float4   discs: centre xyz + radius;   triangles: v0, w = -1
-//   prim_b[p]  float4   discs: unit normal xyz, 0;    triangles: e1 = v1 - v0
-//   prim_c[p]  float4   triangles: e2 = v2 - v0 (unused for discs)
-//   prim_n[p]  float4   unit normal (discs: as given; triangles: from the winding)
-//   nodes[i]   bvh_node 64 B, both children's boxes in the parent
-//   nodes4[i]  bvh4_node 64 B, four quantised child boxes (the traversal tree)
-//   nb_ptr/nb_col       neighbour CSR over absolute prim ids (discs)
-//   acc_*[p]            64-bit accumulators
+//   prims[4p + k]  float4  one 64-byte record per primitive, so that a
+//                  primitive test, a neighbour test and a surface hit each
+//                  touch one aligned block:
+//                    k = 0  discs: centre xyz + radius;  triangles: v0, w = -1
+//                    k = 1  discs: unit normal, 0;       triangles: e1 = v1 - v0
+//                    k = 2  discs: unused;               triangles: e2 = v2 - v0
+//                    k = 3  unit normal (discs as given; triangles from the winding)
+//   nodes4[i]      bvh4_node 64 B, four quantised child boxes (the traversal tree)
+//   nodes[i]       bvh_node 64 B, both children's boxes in the parent (the build's tree)
+//   nb_ptr/nb_col  neighbour CSR over absolute prim ids (discs)
+//   acc[4p + k]    u64  k = 0 sum of deposits, 1 sum of squares (fixed point,
+//                  or FP64 bits), 2 deposit count, 3 unused: one 32-byte
+//                  record, so a deposit's three atomics share a cache line
+//   adj[2p + k]    u64  adjoint sums d/ds (k = 0) and d/dn (k = 1), signed fixed point
+//   mom[2p + k]    f64  sums of deposit^3 (k = 0) and deposit^4 (k = 1)
 // Primitives are stored in Morton order of their instance, so neighbouring
 // primitives in space are neighbours in memory (coalesced leaf and
 // accumulator traffic); `perm` maps a stored position back to the input id.
@@ -19,6 +26,9 @@
 #include <stdint.h>
 
 namespace rti_dev {
+
+constexpr int kPrimStride = 4;  // float4 per primitive record
+constexpr int kAccStride = 4;   // u64 per accumulator record
 
 // Internal node of the binary LBVH. Child c's box is
 //   x: (lo_hi_xy[c].x, lo_hi_xy[c].y), y: (lo_hi_xy[c].z, lo_hi_xy[c].w),

@@ -44,21 +44,14 @@ struct tparams {
   const inst_desc* desc;
   uint32_t n_inst;
   uint64_t ray_lo, ray_hi;  // global ray ids traced by this launch
-  const float4* __restrict__ pa;
-  const float4* __restrict__ pb;
-  const float4* __restrict__ pc;
-  const float4* __restrict__ pn;
+  const float4* __restrict__ prims;  // kPrimStride float4 per primitive
   const bvh_node* __restrict__ nodes;
   const bvh4_node* __restrict__ nodes4;
   const uint32_t* __restrict__ nb_ptr;
   const uint32_t* __restrict__ nb_col;
-  unsigned long long* s1;
-  unsigned long long* s2;
-  unsigned long long* cnt;
-  unsigned long long* ds;
-  unsigned long long* dn;
-  double* s3;  // third and fourth power sums (variance of the variance), or null
-  double* s4;
+  unsigned long long* acc;  // kAccStride per primitive: sum, sum of squares, count
+  unsigned long long* adj;  // 2 per primitive: d/ds, d/dn (adjoint only)
+  double* mom;              // 2 per primitive: sums of deposit^3, ^4 (or null)
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
@@ -90,8 +83,9 @@ template <bool COUNT>
 __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3 d, uint32_t p, trav_state& T,
                                           counters& C) {
   if (COUNT) C.prims++;
-  const float4 A = P.pa[p];
-  const float4 B = P.pb[p];
+  const float4* rec = P.prims + (size_t)kPrimStride * p;
+  const float4 A = rec[0];
+  const float4 B = rec[1];
   float th;
   bool ok;
   if (A.w > 0.0f) {
@@ -105,7 +99,7 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
     ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
   } else {
     // triangle: Moller-Trumbore
-    const float4 C2 = P.pc[p];
+    const float4 C2 = rec[2];
     const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
     const f3 pv = cross(d, e2);
     const float det = dot(e1, pv);
@@ -384,27 +378,28 @@ __device__ __forceinline__ void deposit(const tparams& P, uint32_t p, float drop
                                         counters& C) {
   if (COUNT) C.deps++;
   const double v = (double)drop;
+  unsigned long long* a = P.acc + (size_t)kAccStride * p;  // one 32-byte record
   if (FP64ACC) {
-    unsafeAtomicAdd(reinterpret_cast<double*>(P.s1) + p, v);
-    unsafeAtomicAdd(reinterpret_cast<double*>(P.s2) + p, v * v);
+    unsafeAtomicAdd(reinterpret_cast<double*>(a), v);
+    unsafeAtomicAdd(reinterpret_cast<double*>(a + 1), v * v);
   } else {
-    atomicAdd(P.s1 + p, (unsigned long long)__double2ll_rn(v * (double)scale));
-    atomicAdd(P.s2 + p, (unsigned long long)__double2ll_rn(v * v * (double)scale));
+    atomicAdd(a, (unsigned long long)__double2ll_rn(v * (double)scale));
+    atomicAdd(a + 1, (unsigned long long)__double2ll_rn(v * v * (double)scale));
   }
-  atomicAdd(P.cnt + p, 1ull);
-  if (P.s3) {
+  atomicAdd(a + 2, 1ull);
+  if (P.mom) {
     // higher moments span too many orders of magnitude for the fixed-point
     // scale of the first two; they use FP64 atomics (hit_accumulator.hpp:128-138)
     const double v2 = v * v;
-    unsafeAtomicAdd(P.s3 + p, v2 * v);
-    unsafeAtomicAdd(P.s4 + p, v2 * v * v);
+    unsafeAtomicAdd(P.mom + 2 * (size_t)p, v2 * v);
+    unsafeAtomicAdd(P.mom + 2 * (size_t)p + 1, v2 * v * v);
   }
   if (ADJ) {
     // derivative weights can exceed 1 (by about k s / (1 - s)); they use a
     // scale 2^8 smaller than the primal sums
     const double sa = (double)scale * (1.0 / 256.0);
-    atomicAdd(P.ds + p, (unsigned long long)__double2ll_rn((double)dds * sa));
-    atomicAdd(P.dn + p, (unsigned long long)__double2ll_rn((double)ddn * sa));
+    atomicAdd(P.adj + 2 * (size_t)p, (unsigned long long)__double2ll_rn((double)dds * sa));
+    atomicAdd(P.adj + 2 * (size_t)p + 1, (unsigned long long)__double2ll_rn((double)ddn * sa));
   }
 }
 
@@ -501,7 +496,7 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
   }
   const uint32_t prim = T.pbest;
   const float tp = T.tbest;
-  const float4 N4 = P.pn[prim];
+  const float4 N4 = P.prims[(size_t)kPrimStride * prim + 3];
   const f3 n = make(N4.x, N4.y, N4.z);
   const bool disc = D.prim_kind == 0;
   if (dot(R.d, n) > 0.0f) {  // back face: pass through (tracer.hpp:212-227)
@@ -533,8 +528,8 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
       for (int u = 0; u < 4; ++u) {
         if (q + u >= q1) break;
         if (COUNT) C.nbt++;
-        if (local_intersect(R.o, R.d, P.pa[j[u]], P.pb[j[u]]))
-          deposit<ADJ, FP64ACC, COUNT>(P, j[u], drop, scale, dds, ddn, C);
+        const float4* nr = P.prims + (size_t)kPrimStride * j[u];
+        if (local_intersect(R.o, R.d, nr[0], nr[1])) deposit<ADJ, FP64ACC, COUNT>(P, j[u], drop, scale, dds, ddn, C);
       }
     }
   }
@@ -721,21 +716,14 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.n_inst = L.n_inst;
   P.ray_lo = L.ray_lo;
   P.ray_hi = L.ray_hi;
-  P.pa = L.prim_a;
-  P.pb = L.prim_b;
-  P.pc = L.prim_c;
-  P.pn = L.prim_n;
+  P.prims = L.prims;
   P.nodes = L.nodes;
   P.nodes4 = L.nodes4;
   P.nb_ptr = L.nb_ptr;
   P.nb_col = L.nb_col;
-  P.s1 = L.acc_s1;
-  P.s2 = L.acc_s2;
-  P.cnt = L.acc_cnt;
-  P.ds = L.acc_ds;
-  P.dn = L.acc_dn;
-  P.s3 = L.acc_s3;
-  P.s4 = L.acc_s4;
+  P.acc = L.acc;
+  P.adj = L.adj;
+  P.mom = L.mom;
   P.queue = L.queue;
   P.stats = L.stats;
   P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();

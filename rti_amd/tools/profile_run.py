@@ -3,8 +3,9 @@
   python -m rti_amd.tools.profile_run copy  [--mib 1024]
       one device-to-device copy of a buffer of known size (torch's copy
       kernel): the calibration for FETCH_SIZE / WRITE_SIZE;
-  python -m rti_amd.tools.profile_run trace [--rays 1e7] [--launches 2] [--knobs JSON]
-      the benchmark workload (triangle hole, six (s, n) cases) traced as one
+  python -m rti_amd.tools.profile_run trace [--workload hole] [--rays 1e7] [--launches 2] [--knobs JSON]
+      a benchmark workload (default: the triangle hole, six (s, n) cases;
+      --rays per case for the hole, in total otherwise) traced as one
       persistent launch per call, inputs resident; --knobs sets kernel
       parameters (e.g. '{"bvh_width": 2, "lds_stack": 16}').
 Prints the work done so a summary can turn counters into bytes per segment.
@@ -25,6 +26,7 @@ def main():
     ap.add_argument("--rays", type=float, default=1e7)
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--knobs", default="{}")
+    ap.add_argument("--workload", default="hole")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -36,9 +38,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"what": "copy", "bytes_read": n * 4, "bytes_written": n * 4}))
         return
-    from bench import hole_cases
+    from bench import workload
     from rti_amd import Context
-    sc, cases = hole_cases(a.rays)
+    cases, _, _ = workload(a.workload, a.rays)
     ctx = Context(0)
     ctx.batch_set(cases)
     ctx.set_params(**json.loads(a.knobs))
@@ -47,7 +49,8 @@ def main():
     for _ in range(a.launches):
         ctx.batch_run()
         segs.append(int(ctx.counters()[0]))
-    print(json.dumps({"what": "trace", "rays_per_launch": int(a.rays) * len(cases), "segments_per_launch": segs}))
+    print(json.dumps({"what": "trace", "workload": a.workload, "rays_per_launch": int(sum(c["n_rays"] for c in cases)),
+                      "segments_per_launch": segs}))
 
 
 if __name__ == "__main__":

@@ -125,6 +125,21 @@ def test_batch_equals_single_runs_bit_exact():
         assert np.array_equal(plain[k]["raw"], batch[k]["raw"]), k
 
 
+def test_batch_adjoint_equals_single_runs_bit_exact():
+    items = _batch_items()
+    _say("batch adjoint of 4 in one persistent launch")
+    with Context() as ctx:
+        ctx.batch_set(items)
+        ctx.set_params(adjoint=1)
+        ctx.batch_run()
+        got = [ctx.batch_adjoint(i) for i in range(len(items))]
+    for k, it in enumerate(items):
+        kw = {key: v for key, v in it.items() if key != "scene"}
+        single = run_scene(it["scene"], instance=k, adjoint=1, **kw)
+        assert np.array_equal(got[k][0], single["ds"]), k
+        assert np.array_equal(got[k][1], single["dn"]), k
+
+
 # ---------------------------------------------------------------------------
 # adjoint
 # ---------------------------------------------------------------------------
