@@ -181,30 +181,22 @@ def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
     return out
 
 
-def measured_valu_busy(name):
-    """Fraction of cycles the SIMDs issued VALU instructions in the trace
-    kernel, from the committed counter summary (SQ_ACTIVE_INST_VALU), or None."""
+def profile_figure(name, key):
+    """A figure of the committed counter summary of the trace kernel on this
+    workload (a line "key: value" or "... key = value"), or None without one.
+    Keys: "bytes_per_segment" (HBM-side traffic from FETCH_SIZE and WRITE_SIZE,
+    each scaled by what the counter reads for a copy kernel of known size),
+    "l2_hit_rate" (TCC_HIT / (TCC_HIT + TCC_MISS)), "/ busy cycles"
+    (SQ_ACTIVE_INST_VALU over the SIMDs' busy cycles)."""
     path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if "/ busy cycles =" in line:
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1].split()[0])
+            if key.startswith("/") and key in line and "=" in line:
                 return float(line.split("=")[-1])
-    return None
-
-
-def measured_traffic(name, segments_per_launch):
-    """HBM-side traffic per launch from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE of the trace kernel, each scaled by what the
-    counter reads for a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("bytes_per_segment:"):
-                return float(line.split(":")[1].split()[0]) * segments_per_launch
     return None
 
 
@@ -293,12 +285,18 @@ def main():
     B, parts = bytes_per_segment(cnt_ctx.counters(), prim_bytes, 80.0 if adj else 48.0)
     cnt_ctx.close()
     achieved = value * B / 1e9
+    b_meas = profile_figure(args.workload, "bytes_per_segment")
+    traffic = None if b_meas is None else b_meas * segments
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload, segments),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_gbs": None if traffic is None else round(traffic / (ms_per_step / 1e3) / 1e9, 1),
+                "l2_hit_rate": profile_figure(args.workload, "l2_hit_rate"),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
-                "valu_busy": measured_valu_busy(args.workload),
-                "note": "the trace kernel is limited by VALU issue (valu_busy, from the committed SQ counter "
-                        "summary), not by HBM bandwidth; frac is algorithmic bytes over the HBM peak"}
+                "valu_busy": profile_figure(args.workload, "/ busy cycles"),
+                "note": "achieved and frac: algorithmic bytes per segment x segments/s over the HBM peak; "
+                        "traffic, traffic_gbs, l2_hit_rate and valu_busy: from the committed counter summary "
+                        "of the trace kernel on this workload (null without one). A valu_busy near 1 with "
+                        "traffic_gbs well below the peak means VALU issue, not HBM, binds the kernel"}
 
     cpu = None
     if not args.no_cpu:
