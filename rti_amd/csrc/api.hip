@@ -98,6 +98,7 @@ struct rti_ctx {
   bool have_results = false;
   // scene store
   dbuf prims, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
+  uint32_t prim_shift = rti_dev::kPrimShiftMixed;  // log2 float4 per primitive record
   dbuf acc, adj, mom, queue, stats;  // accumulator records (rti_device.hpp)
   // finalisation scratch
   dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj, out_err;
@@ -304,7 +305,11 @@ int prepare(rti_ctx* ctx) {
   if (np >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives in total");
   ctx->n_total = np;
   ctx->nodes_total = nn;
-  HIP_TRY(ctx, ctx->prims.ensure(np * 16 * rti_dev::kPrimStride));
+  // 32-byte records when every primitive is a disc, else 64-byte records
+  bool discs_only = true;
+  for (auto& I : ctx->inst) discs_only = discs_only && I.kind == 0;
+  ctx->prim_shift = discs_only ? rti_dev::kPrimShiftDiscs : rti_dev::kPrimShiftMixed;
+  HIP_TRY(ctx, ctx->prims.ensure((np * 16) << ctx->prim_shift));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
   HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
@@ -314,7 +319,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->queue.ensure(64));
   HIP_TRY(ctx, ctx->stats.ensure(64));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
-  rti_impl::store_ptrs st{ctx->prims.as<float4>(), ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
+  rti_impl::store_ptrs st{ctx->prims.as<float4>(), ctx->prim_shift, ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
                           ctx->nb_ptr.as<uint32_t>(), nullptr, ctx->areas.as<float>()};
   HIP_TRY(ctx, hipError_t(rti_impl::scratch_reserve(ctx->scratch, (uint32_t)maxn)));
   dbuf in_a, in_b;
@@ -508,6 +513,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.desc = ctx->desc.as<inst_desc>();
   L.n_inst = (uint32_t)ctx->inst.size();
   L.prims = ctx->prims.as<float4>();
+  L.prim_shift = ctx->prim_shift;
   L.nodes = ctx->nodes.as<bvh_node>();
   L.nodes4 = ctx->nodes4.as<bvh4_node>();
   L.nb_ptr = ctx->nb_ptr.as<uint32_t>();
@@ -978,8 +984,8 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
   }
   if (perm) HIP_TRY(ctx, hipMemcpy(perm, ctx->perm.as<uint32_t>() + I.prim_begin, I.n * 4, hipMemcpyDeviceToHost));
   if (prim_a)  // the first float4 of each primitive record
-    HIP_TRY(ctx, hipMemcpy2D(prim_a, 16, ctx->prims.as<float4>() + (size_t)rti_dev::kPrimStride * I.prim_begin,
-                             16 * rti_dev::kPrimStride, 16, I.n, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy2D(prim_a, 16, ctx->prims.as<float4>() + ((size_t)I.prim_begin << ctx->prim_shift),
+                             (size_t)16 << ctx->prim_shift, 16, I.n, hipMemcpyDeviceToHost));
   return RTI_OK;
 }
 

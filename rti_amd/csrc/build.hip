@@ -209,13 +209,13 @@ __global__ void k_morton(setup_args a, uint32_t* keys, uint32_t* vals) {
 
 // Stored primitive records in Morton order, the permutation, and leaf boxes.
 __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t prim_begin, float4* prims,
-                                uint32_t* perm, float4* leaf_lo, float4* leaf_hi) {
+                                uint32_t shift, uint32_t* perm, float4* leaf_lo, float4* leaf_hi) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.n) return;
   uint32_t i = order[j];
   size_t g = (size_t)prim_begin + j;
   perm[g] = i;
-  float4* rec = prims + kPrimStride * g;
+  float4* rec = prims + (g << shift);
   float lo[3], hi[3];
   if (a.kind == 0) {
     float4 c = make_float4(a.xyzr[4 * (size_t)i], a.xyzr[4 * (size_t)i + 1], a.xyzr[4 * (size_t)i + 2],
@@ -223,8 +223,10 @@ __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t pr
     float n0 = a.nrm[3 * (size_t)i], n1 = a.nrm[3 * (size_t)i + 1], n2 = a.nrm[3 * (size_t)i + 2];
     rec[0] = c;
     rec[1] = make_float4(n0, n1, n2, 0.0f);
-    rec[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    rec[3] = make_float4(n0, n1, n2, 0.0f);
+    if (shift == kPrimShiftMixed) {
+      rec[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      rec[3] = make_float4(n0, n1, n2, 0.0f);
+    }
     float nn[3] = {n0, n1, n2};
     float cc[3] = {c.x, c.y, c.z};
 #pragma unroll
@@ -576,12 +578,12 @@ __device__ __forceinline__ bool box_overlap(float lox, float hix, float loy, flo
 }
 
 template <bool kWrite>
-__global__ void k_neighbors(const float4* prims, uint32_t prim_begin, uint32_t n, uint32_t root, const bvh_node* nodes,
+__global__ void k_neighbors(const float4* prims, uint32_t shift, uint32_t prim_begin, uint32_t n, uint32_t root, const bvh_node* nodes,
                             float rmax, uint32_t* counts, const uint32_t* row_ptr, uint32_t* col) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint32_t self = prim_begin + j;
-  float4 p = prims[kPrimStride * (size_t)self];
+  float4 p = prims[(size_t)self << shift];
   float reach = p.w + rmax;
   float3 qlo = make_float3(p.x - reach, p.y - reach, p.z - reach);
   float3 qhi = make_float3(p.x + reach, p.y + reach, p.z + reach);
@@ -605,7 +607,7 @@ __global__ void k_neighbors(const float4* prims, uint32_t prim_begin, uint32_t n
       if (!h) continue;
       if (cc < 0) {
         uint32_t q = (uint32_t)(~cc);
-        if (q != self && overlap_pred(p, prims[kPrimStride * (size_t)q])) {
+        if (q != self && overlap_pred(p, prims[(size_t)q << shift])) {
           if (kWrite) col[w] = q;
           ++w;
           ++cnt;
@@ -701,16 +703,16 @@ __device__ float disc_area_inside(float xmin, float ymin, float xmax, float ymax
   return full - area;
 }
 
-__global__ void k_areas(int kind, const float4* prims, uint32_t prim_begin,
+__global__ void k_areas(int kind, const float4* prims, uint32_t shift, uint32_t prim_begin,
                         uint32_t n, float xmin, float ymin, float xmax, float ymax, float* areas) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   size_t g = (size_t)prim_begin + j;
   float A;
   if (kind == 0) {
-    A = disc_area_inside(xmin, ymin, xmax, ymax, prims[kPrimStride * g], prims[kPrimStride * g + 1]);
+    A = disc_area_inside(xmin, ymin, xmax, ymax, prims[g << shift], prims[(g << shift) + 1]);
   } else {
-    float4 e1 = prims[kPrimStride * g + 1], e2 = prims[kPrimStride * g + 2];
+    float4 e1 = prims[(g << shift) + 1], e2 = prims[(g << shift) + 2];
     f3 c = cross(make(e1.x, e1.y, e1.z), make(e2.x, e2.y, e2.z));
     A = 0.5f * sqrtf(dot(c, c));
   }
@@ -802,7 +804,7 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
                                                      shift, s.hist, nblocks);
     cur ^= 1;
   }
-  k_scatter_prims<<<blocks_for(n, T), T, 0, stream>>>(a, s.vals[cur], job.prim_begin, st.prims, st.perm, s.leaf_lo,
+  k_scatter_prims<<<blocks_for(n, T), T, 0, stream>>>(a, s.vals[cur], job.prim_begin, st.prims, st.prim_shift, st.perm, s.leaf_lo,
                                                        s.leaf_hi);
   e = hipMemsetAsync(s.depth, 0, 4, stream);
   if (e != hipSuccess) return e;
@@ -816,7 +818,7 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
     k_refit<<<blocks_for(n, T), T, 0, stream>>>((int)n, job.prim_begin, job.node_begin, st.nodes, s.parent, s.flags,
                                                 s.leaf_lo, s.leaf_hi, s.depth);
   }
-  k_areas<<<blocks_for(n, T), T, 0, stream>>>(job.prim_kind, st.prims, job.prim_begin, n,
+  k_areas<<<blocks_for(n, T), T, 0, stream>>>(job.prim_kind, st.prims, st.prim_shift, job.prim_begin, n,
                                               job.area_box[0], job.area_box[1], job.area_box[2], job.area_box[3],
                                               st.areas);
   e = hipGetLastError();
@@ -867,14 +869,14 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
 
 hipError_t neighbor_count(const build_job& job, const store_ptrs& st, build_scratch& s, hipStream_t stream) {
   const unsigned T = 128;
-  k_neighbors<false><<<blocks_for(job.n, T), T, 0, stream>>>(st.prims, job.prim_begin, job.n, job.node_begin,
+  k_neighbors<false><<<blocks_for(job.n, T), T, 0, stream>>>(st.prims, st.prim_shift, job.prim_begin, job.n, job.node_begin,
                                                               st.nodes, job.rmax, s.counts, nullptr, nullptr);
   return hipGetLastError();
 }
 
 hipError_t neighbor_fill(const build_job& job, const store_ptrs& st, hipStream_t stream) {
   const unsigned T = 128;
-  k_neighbors<true><<<blocks_for(job.n, T), T, 0, stream>>>(st.prims, job.prim_begin, job.n, job.node_begin,
+  k_neighbors<true><<<blocks_for(job.n, T), T, 0, stream>>>(st.prims, st.prim_shift, job.prim_begin, job.n, job.node_begin,
                                                              st.nodes, job.rmax, nullptr, st.nb_ptr, st.nb_col);
   return hipGetLastError();
 }

@@ -28,7 +28,8 @@ struct build_job {
 
 // Scene store (absolute indexing; see rti_device.hpp).
 struct store_ptrs {
-  float4* prims;            // kPrimStride float4 per primitive (see rti_device.hpp)
+  float4* prims;            // 1 << prim_shift float4 per primitive (see rti_device.hpp)
+  uint32_t prim_shift;      // kPrimShiftDiscs or kPrimShiftMixed
   uint32_t* perm;           // stored position -> input id (per instance)
   rti_dev::bvh_node* nodes;
   rti_dev::bvh4_node* nodes4;  // four-wide traversal tree, same slot ranges as nodes
@@ -78,6 +79,7 @@ struct trace_launch {
   uint32_t n_inst;
   uint64_t ray_lo, ray_hi;          // global ray ids [ray_lo, ray_hi) traced by this launch
   const float4* prims;
+  uint32_t prim_shift;              // log2 float4 per primitive record
   const rti_dev::bvh_node* nodes;
   const rti_dev::bvh4_node* nodes4;
   const uint32_t* nb_ptr;

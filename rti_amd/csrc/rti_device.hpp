@@ -2,13 +2,17 @@
 //
 // Layout in HBM (one "scene store" holds one geometry or a whole batch; all
 // arrays are concatenated over the instances and indexed by absolute ids):
-//   prims[4p + k]  float4  one 64-byte record per primitive, so that a
+//   prims[(p << s) + k]  float4  one record per primitive, so that a
 //                  primitive test, a neighbour test and a surface hit each
-//                  touch one aligned block:
+//                  touch one aligned block. A store of discs only has
+//                  32-byte records (s = 1), any other store 64-byte records
+//                  (s = 2):
 //                    k = 0  discs: centre xyz + radius;  triangles: v0, w = -1
 //                    k = 1  discs: unit normal, 0;       triangles: e1 = v1 - v0
 //                    k = 2  discs: unused;               triangles: e2 = v2 - v0
 //                    k = 3  unit normal (discs as given; triangles from the winding)
+//                  The last float4 of a record, k = (1 << s) - 1, is always
+//                  the unit normal.
 //   nodes4[i]      bvh4_node 64 B, four quantised child boxes (the traversal tree)
 //   nodes[i]       bvh_node 64 B, both children's boxes in the parent (the build's tree)
 //   nb_ptr/nb_col  neighbour CSR over absolute prim ids (discs)
@@ -27,7 +31,8 @@
 
 namespace rti_dev {
 
-constexpr int kPrimStride = 4;  // float4 per primitive record
+constexpr uint32_t kPrimShiftDiscs = 1;  // log2 float4 per record, store of discs only
+constexpr uint32_t kPrimShiftMixed = 2;  // log2 float4 per record, any triangle in the store
 constexpr int kAccStride = 4;   // u64 per accumulator record
 
 // Internal node of the binary LBVH. Child c's box is

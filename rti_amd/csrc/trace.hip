@@ -44,7 +44,8 @@ struct tparams {
   const inst_desc* desc;
   uint32_t n_inst;
   uint64_t ray_lo, ray_hi;  // global ray ids traced by this launch
-  const float4* __restrict__ prims;  // kPrimStride float4 per primitive
+  const float4* __restrict__ prims;  // 1 << prim_shift float4 per primitive
+  uint32_t prim_shift;
   const bvh_node* __restrict__ nodes;
   const bvh4_node* __restrict__ nodes4;
   const uint32_t* __restrict__ nb_ptr;
@@ -83,7 +84,7 @@ template <bool COUNT>
 __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3 d, uint32_t p, trav_state& T,
                                           counters& C) {
   if (COUNT) C.prims++;
-  const float4* rec = P.prims + (size_t)kPrimStride * p;
+  const float4* rec = P.prims + ((size_t)p << P.prim_shift);
   const float4 A = rec[0];
   const float4 B = rec[1];
   float th;
@@ -496,7 +497,8 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
   }
   const uint32_t prim = T.pbest;
   const float tp = T.tbest;
-  const float4 N4 = P.prims[(size_t)kPrimStride * prim + 3];
+  // the record's last float4 is the unit normal
+  const float4 N4 = P.prims[((size_t)(prim + 1) << P.prim_shift) - 1];
   const f3 n = make(N4.x, N4.y, N4.z);
   const bool disc = D.prim_kind == 0;
   if (dot(R.d, n) > 0.0f) {  // back face: pass through (tracer.hpp:212-227)
@@ -528,7 +530,7 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
       for (int u = 0; u < 4; ++u) {
         if (q + u >= q1) break;
         if (COUNT) C.nbt++;
-        const float4* nr = P.prims + (size_t)kPrimStride * j[u];
+        const float4* nr = P.prims + ((size_t)j[u] << P.prim_shift);
         if (local_intersect(R.o, R.d, nr[0], nr[1])) deposit<ADJ, FP64ACC, COUNT>(P, j[u], drop, scale, dds, ddn, C);
       }
     }
@@ -717,6 +719,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.ray_lo = L.ray_lo;
   P.ray_hi = L.ray_hi;
   P.prims = L.prims;
+  P.prim_shift = L.prim_shift;
   P.nodes = L.nodes;
   P.nodes4 = L.nodes4;
   P.nb_ptr = L.nb_ptr;
