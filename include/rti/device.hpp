@@ -22,6 +22,10 @@
 //     ::specular: the GPU implements those two laws;
 //   * errors are reported by exceptions (std::runtime_error) instead of
 //     asserts.
+// Extension (no public counterpart in the reference):
+//   * set_circular_source(true) starts the primary rays on the disc of
+//     device::create_circular_source_from_bounding_box (device.hpp:269-280,
+//     ray/disc_origin_z.hpp:17-33) instead of the rectangle run() uses.
 #pragma once
 
 #include <array>
@@ -62,6 +66,7 @@ class device final {
   void set_x(bound_condition cond) { xCond = cond; }
   void set_y(bound_condition cond) { yCond = cond; }
   void set(ray::i_direction<numeric_type>& srcDirection) { cosExponent = srcDirection.cos_exponent(); }
+  void set_circular_source(bool circular) { circularSource = circular; }
 
   void run() {
     if (points.size() != normals.size() || points.size() != spacing.size())
@@ -86,6 +91,7 @@ class device final {
     p.bc_x = xCond == bound_condition::PERIODIC ? RTI_BC_PERIODIC : RTI_BC_REFLECTIVE;
     p.bc_y = yCond == bound_condition::PERIODIC ? RTI_BC_PERIODIC : RTI_BC_REFLECTIVE;
     p.cos_exponent = (float)cosExponent;
+    p.source = circularSource ? RTI_SOURCE_DISC : RTI_SOURCE_RECTANGLE;
     p.sticking = (float)sticking_probability();
     p.diffuse_prob = std::is_base_of<reflection::diffuse<numeric_type>, reflection_type>::value ? 1.0f : 0.0f;
     check(rti_set_params(ctx, &p), "rti_set_params");
@@ -146,6 +152,7 @@ class device final {
   bound_condition xCond = geo::bound_condition::REFLECTIVE;
   bound_condition yCond = geo::bound_condition::REFLECTIVE;
   numeric_type cosExponent = 1;
+  bool circularSource = false;
   rti_ctx* ctx = nullptr;
 };
 }  // namespace rti

@@ -75,6 +75,14 @@ enum rti_kernel {
   RTI_KERNEL_COUNT = 2        /* persistent with node / primitive / neighbour counters  */
 };
 enum rti_accum { RTI_ACCUM_FIXED64 = 0, RTI_ACCUM_FP64 = 1 };
+/* Where primary rays start on the source plane. RECTANGLE: uniform over the
+ * boundary box (rectangle_origin_z, src/rti/ray/rectangle_origin_z.hpp:23-37,
+ * as placed by device::create_rectangular_source_from_bounding_box,
+ * src/rti/device.hpp:257-267; what device::run uses). DISC: uniform over the
+ * disc centred on the box with radius half its x extent (disc_origin_z,
+ * src/rti/ray/disc_origin_z.hpp:17-33, as placed by
+ * device::create_circular_source_from_bounding_box, device.hpp:269-280). */
+enum rti_source { RTI_SOURCE_RECTANGLE = 0, RTI_SOURCE_DISC = 1 };
 
 typedef struct rti_params {
   uint64_t n_rays;        /* primary rays (reference default 1024, device.hpp:289)    */
@@ -100,6 +108,7 @@ typedef struct rti_params {
                              2 = binary LBVH                                           */
   int32_t moments;        /* 1 = also accumulate sums of deposit^3 and deposit^4 (FP64)
                              for the variance of the variance                          */
+  int32_t source;         /* rti_source (default RTI_SOURCE_RECTANGLE)                  */
 } rti_params;
 
 typedef struct rti_stats {

@@ -74,7 +74,7 @@ class Scene(ctypes.Structure):
         ("n_prims", ctypes.c_uint64),
         ("xyzr", _fp), ("nrm", _fp),
         ("verts", _fp), ("n_verts", ctypes.c_uint64), ("tris", _u32p),
-        ("nb_ptr", _u32p), ("nb_col", _u32p),
+        ("nb_ptr", _u32p), ("nb_col", _u32p), ("tie_rank", _u32p),
     ]
 
 
@@ -88,6 +88,7 @@ class Params(ctypes.Structure):
         ("roulette", ctypes.c_int32), ("fp64", ctypes.c_int32),
         ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float),
         ("threads", ctypes.c_int32), ("adjoint", ctypes.c_int32), ("moments", ctypes.c_int32),
+        ("source", ctypes.c_int32),
     ]
 
 
@@ -112,6 +113,8 @@ def _declare(L):
     L.orc_smooth_normalize.argtypes = [_dp, _fp, _u32p, _u32p, ctypes.c_uint64, _fp]
     L.orc_trace.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.POINTER(Result)]
     L.orc_source_dir.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+    L.orc_source_origins.argtypes = [ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_uint64, _fp]
+    L.orc_source_origins.restype = None
     L.orc_diffuse_dir.argtypes = [_fp, ctypes.c_float, ctypes.c_float, _fp]
     L.orc_roulette.argtypes = [_fp, ctypes.c_float]
     L.orc_roulette.restype = ctypes.c_int
@@ -154,6 +157,19 @@ def neighbors(xyzr):
     return row, col[: int(e)]
 
 
+def source_origins(box, src_z, n, source=0, seed=1, instance=0, ray_begin=0):
+    """Primary-ray origins (n x 3, float32) of rays [ray_begin, ray_begin + n):
+    rectangle over the box (source 0) or the disc inscribed in it (source 1)."""
+    p = Params()
+    p.seed, p.instance, p.source = int(seed), int(instance), int(source)
+    for i in range(6):
+        p.box[i] = float(box[i])
+    p.src_z = float(src_z)
+    out = np.zeros((int(n), 3), dtype=np.float32)
+    lib().orc_source_origins(ctypes.byref(p), int(ray_begin), int(n), _ptr(out, _fp))
+    return out
+
+
 def source_dir(r1, r2, n):
     out = np.zeros(3, dtype=np.float32)
     lib().orc_source_dir(float(r1), float(r2), float(n), _ptr(out, _fp))
@@ -186,11 +202,13 @@ def smooth_normalize(values, areas, row, col):
 
 def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc_y=0, cos_exponent=1.0,
           diffuse_prob=1.0, tnear=1e-4, roulette=True, fp64=False, threads=0, ray_begin=0, adjoint=False,
-          neighbors=None, moments=False):
+          neighbors=None, moments=False, source=0, tie_rank=None):
     """Run the oracle trace. ``scene`` is a dict from ``rti_amd.geometry``.
 
     Returns a dict with s1, s2, cnt (per primitive), s3 and s4 with
-    ``moments``, and the counters.
+    ``moments``, and the counters. ``tie_rank`` (one entry per primitive):
+    of hits at exactly equal distance the primitive of lower rank is the
+    closest (default: lower index).
     """
     keep = []
     sc = Scene()
@@ -220,6 +238,11 @@ def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc
         sc.n_verts = verts.shape[0]
         sc.tris = _ptr(tris, _u32p)
     sc.n_prims = n
+    if tie_rank is not None:
+        tr = np.ascontiguousarray(tie_rank, dtype=np.uint32)
+        assert tr.shape == (n,)
+        keep.append(tr)
+        sc.tie_rank = _ptr(tr, _u32p)
     p = Params()
     p.n_rays = int(n_rays)
     p.ray_begin = int(ray_begin)
@@ -238,6 +261,7 @@ def trace(scene, *, n_rays, box, src_z, sticking, seed=1, instance=0, bc_x=0, bc
     p.threads = int(threads)
     p.adjoint = 1 if adjoint else 0
     p.moments = 1 if moments else 0
+    p.source = int(source)
     out = {"s1": np.zeros(n), "s2": np.zeros(n), "cnt": np.zeros(n, dtype=np.uint64)}
     if moments:
         out["s3"] = np.zeros(n)

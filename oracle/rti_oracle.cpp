@@ -353,6 +353,9 @@ struct orc_scene {
   const uint32_t* tris;     // triangles: n_prims * 3
   const uint32_t* nb_ptr;   // neighbour CSR row pointers (n_prims + 1), discs only; may be null
   const uint32_t* nb_col;   // neighbour CSR columns
+  const uint32_t* tie_rank; // optional: of hits at equal distance the primitive of lower rank
+                            // wins; null = lower index. Embree leaves such ties to its
+                            // traversal order; coplanar overlapping discs make them common.
 };
 
 struct orc_params {
@@ -372,6 +375,7 @@ struct orc_params {
   int32_t threads;          // 0 = OpenMP default
   int32_t adjoint;          // 1 = also accumulate d/ds and d/dn estimators
   int32_t moments;          // 1 = also accumulate sums of deposit^3 and deposit^4
+  int32_t source;           // 0 = rectangle over the box, 1 = disc inscribed in it
 };
 
 struct orc_result {
@@ -504,6 +508,7 @@ struct scene_view {
   std::vector<v3<R>> e1, e2;
   const uint32_t* nb_ptr;
   const uint32_t* nb_col;
+  const uint32_t* rank;   // tie order (null = index order)
   bvh tree;
 };
 
@@ -511,6 +516,7 @@ template <typename R>
 void make_scene(const orc_scene* s, scene_view<R>& v) {
   v.kind = s->prim_kind;
   v.n = s->n_prims;
+  v.rank = s->tie_rank;
   v.nb_ptr = s->nb_ptr;
   v.nb_col = s->nb_col;
   v.c.resize(v.n);
@@ -602,7 +608,7 @@ inline bool closest_hit(const scene_view<R>& s, v3<R> o, v3<R> d, R tnear, R& tb
         th = dot(s.e2[p], qv) * idet;
         if (!(th >= tnear && th <= tbest)) continue;
       }
-      if (th < tbest || (th == tbest && (!found || p < pbest))) {
+      if (th < tbest || (th == tbest && (!found || (s.rank ? s.rank[p] < s.rank[pbest] : p < pbest)))) {
         tbest = th;
         pbest = p;
         found = true;
@@ -697,6 +703,30 @@ struct thread_acc {
 };
 
 template <typename R>
+v3<R> source_origin(const orc_params& p, const u32x4& u, uint32_t clo, uint32_t chi) {
+  const R xmin = p.box[0], xmax = p.box[1], ymin = p.box[2], ymax = p.box[3];
+  if (p.source == 0) {  // rectangle_origin_z::get (src/rti/ray/rectangle_origin_z.hpp:23-37)
+    return {xmin + (xmax - xmin) * (R)u01(u.v[0]), ymin + (ymax - ymin) * (R)u01(u.v[1]), (R)p.src_z};
+  }
+  // disc_origin_z::get (src/rti/ray/disc_origin_z.hpp:17-33) on the disc of
+  // device::create_circular_source_from_bounding_box (src/rti/device.hpp:269-280):
+  // centre of the box, radius half its x extent; offsets uniform in [-r, r]^2,
+  // redrawn until inside the disc. Attempt a > 0 draws the Philox block
+  // (ray, 0, a); after 64 rejections the ray starts at the centre.
+  const R cx = (xmin + xmax) * (R)0.5, cy = (ymin + ymax) * (R)0.5, rad = (xmax - xmin) * (R)0.5;
+  uint32_t ux = u.v[0], uy = u.v[1];
+  for (uint32_t a = 1;; ++a) {
+    const R dx = ((R)2 * (R)u01(ux) - (R)1) * rad;
+    const R dy = ((R)2 * (R)u01(uy) - (R)1) * rad;
+    if (std::sqrt(std::fma(dx, dx, dy * dy)) <= rad) return {cx + dx, cy + dy, (R)p.src_z};
+    if (a == 64) return {cx, cy, (R)p.src_z};
+    const u32x4 v = philox({{clo, chi, 0u, a}}, p.seed, p.instance);
+    ux = v.v[0];
+    uy = v.v[1];
+  }
+}
+
+template <typename R>
 void trace_range(const scene_view<R>& s, const orc_params& p, thread_acc& acc, uint64_t r0, uint64_t r1) {
   const R tnear = (R)p.tnear;
   const R xmin = p.box[0], xmax = p.box[1], ymin = p.box[2], ymax = p.box[3];
@@ -719,8 +749,7 @@ void trace_range(const scene_view<R>& s, const orc_params& p, thread_acc& acc, u
     const uint32_t clo = (uint32_t)ray, chi = (uint32_t)(ray >> 32);
     u32x4 u = philox({{clo, chi, 0u, 0u}}, p.seed, p.instance);
     uint32_t event = 1;
-    // rectangle_origin_z::get (src/rti/ray/rectangle_origin_z.hpp:23-37)
-    v3<R> o = {xmin + (xmax - xmin) * (R)u01(u.v[0]), ymin + (ymax - ymin) * (R)u01(u.v[1]), (R)p.src_z};
+    v3<R> o = source_origin<R>(p, u, clo, chi);
     float lncos;
     v3<R> d = source_dir<R>(u01(u.v[2]), u01_open0(u.v[3]), nexp, lncos);
     const double score_n = 1.0 / (nexp + 1.0) + (double)lncos;
@@ -852,6 +881,17 @@ int trace_impl(const orc_scene* sc, const orc_params* p, orc_result* res) {
 extern "C" {
 
 // Samplers, exposed for the distribution and CRN tests.
+// Primary-ray origins of rays [r0, r0 + n) (xyz per ray) for params p.
+void orc_source_origins(const orc_params* p, uint64_t r0, uint64_t n, float* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t ray = r0 + i;
+    const uint32_t clo = (uint32_t)ray, chi = (uint32_t)(ray >> 32);
+    const u32x4 u = philox({{clo, chi, 0u, 0u}}, p->seed, p->instance);
+    const v3<float> o = source_origin<float>(*p, u, clo, chi);
+    out[3 * i] = o.x; out[3 * i + 1] = o.y; out[3 * i + 2] = o.z;
+  }
+}
+
 void orc_source_dir(float r1, float r2, float n, float out[3]) {
   float lncos;
   v3<float> d = source_dir<float>(r1, r2, n, lncos);

@@ -26,6 +26,8 @@ KERNEL_PLAIN = 1
 KERNEL_COUNT = 2
 ACCUM_FIXED64 = 0
 ACCUM_FP64 = 1
+SOURCE_RECTANGLE = 0
+SOURCE_DISC = 1
 
 # Every symbol the header declares; tests check the library exports them all.
 EXPORTS = [
@@ -52,7 +54,7 @@ class Params(ctypes.Structure):
         ("roulette", ctypes.c_int32), ("src_z", ctypes.c_float), ("kernel", ctypes.c_int32),
         ("adjoint", ctypes.c_int32), ("accum", ctypes.c_int32), ("block_size", ctypes.c_int32),
         ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
-        ("bvh_width", ctypes.c_int32), ("moments", ctypes.c_int32),
+        ("bvh_width", ctypes.c_int32), ("moments", ctypes.c_int32), ("source", ctypes.c_int32),
     ]
 
 
@@ -268,6 +270,14 @@ class Context:
                                             ctypes.byref(nn), _ptr(pa, _fp)))
         return nodes, perm, pa
 
+    def perm(self):
+        """Stored position -> input id of the (first) geometry: the library's
+        order, which also breaks exact distance ties (lower position wins)."""
+        nn = ctypes.c_uint64(0)
+        perm = np.zeros(self.n, dtype=np.uint32)
+        self._check(lib().rti_get_hierarchy(self._h, None, 0, _ptr(perm, _u32p), ctypes.byref(nn), None))
+        return perm
+
     def wide_hierarchy(self):
         """Structured array of the four-wide traversal tree (root first)."""
         nn = ctypes.c_uint64(0)
@@ -379,6 +389,7 @@ def run_scene(scene, device=0, **params):
         out["areas"] = ctx.areas()
         out["stats"] = ctx.stats()
         out["counters"] = ctx.counters()
+        out["perm"] = ctx.perm()
         if params.get("adjoint"):
             out["ds"], out["dn"] = ctx.adjoint()
         out["rel_err"], out["vov"] = ctx.error_stats(vov=bool(params.get("moments")))

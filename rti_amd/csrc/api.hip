@@ -156,6 +156,8 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
     return fail(ctx, RTI_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
   if (p.moments != 0 && p.moments != 1) return fail(ctx, RTI_ERR_INVALID_ARG, "moments must be 0 or 1");
+  if (p.source != RTI_SOURCE_RECTANGLE && p.source != RTI_SOURCE_DISC)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "source must be RTI_SOURCE_RECTANGLE or RTI_SOURCE_DISC");
   return RTI_OK;
 }
 
@@ -449,6 +451,7 @@ int upload_descriptors(rti_ctx* ctx) {
     d.prim_kind = I.kind;
     d.root = (int32_t)I.node_begin;
     d.root4 = (int32_t)I.node_begin;
+    d.source = p.source;
     d.ray_begin = rb;
     d.n_rays = p.n_rays;
     d.seed = p.seed;

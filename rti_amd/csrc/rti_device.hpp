@@ -75,7 +75,7 @@ struct __align__(16) inst_desc {
   uint32_t seed, key1, roulette, prim_begin;
   uint32_t n_prims, pad0, pad1, pad2;
   float score_c, fx_scale;          // score_c = 1 / (n + 1); fixed-point scale 2^shift
-  int32_t root4, pad5;              // root4: absolute four-wide node index
+  int32_t root4, source;            // root4: absolute four-wide node index; source: rti_source
 };
 static_assert(sizeof(inst_desc) == 128, "descriptor size");
 

@@ -323,24 +323,34 @@ struct lane_stack<LS, 4> {
 // Lateral boundary faces x = xmin/xmax, y = ymin/ymax over z in [zmin, zmax]
 // (src/rti/geo/boundary_x_y.hpp:147-235; no top or bottom face).
 // Face ids: 0 xmax, 1 ymin, 2 xmin, 3 ymax; -1 none.
+// All four faces are tested, as a closest-hit query against the eight
+// boundary triangles does: an origin may lie outside the box (a disc-source
+// origin beyond a box narrower in y than in x, a reflection off a disc that
+// sticks out of it, a reflective bounce rounded just past the face), where
+// the face behind the direction's sign or the one beyond the nearer plane
+// can be the first one at t >= tnear. Later faces win ties (t <= best).
+__device__ __forceinline__ void face_test(int f, float t, float a, float alo, float ahi, float z,
+                                          const inst_desc& D, float tnear, float& best, int& face) {
+  if (t >= tnear && t <= best && a >= alo && a <= ahi && z >= D.zmin && z <= D.zmax) {
+    best = t;
+    face = f;
+  }
+}
+
 __device__ __forceinline__ int boundary_hit(const inst_desc& D, f3 o, f3 d, float tnear, float& tb) {
   int face = -1;
   float best = FLT_MAX;
   if (d.x != 0.0f) {
-    const float tx = ((d.x > 0.0f ? D.xmax : D.xmin) - o.x) / d.x;
-    const float y = fmaf(d.y, tx, o.y), z = fmaf(d.z, tx, o.z);
-    if (tx >= tnear && y >= D.ymin && y <= D.ymax && z >= D.zmin && z <= D.zmax) {
-      best = tx;
-      face = d.x > 0.0f ? 0 : 2;
-    }
+    float t = (D.xmax - o.x) / d.x;
+    face_test(0, t, fmaf(d.y, t, o.y), D.ymin, D.ymax, fmaf(d.z, t, o.z), D, tnear, best, face);
+    t = (D.xmin - o.x) / d.x;
+    face_test(2, t, fmaf(d.y, t, o.y), D.ymin, D.ymax, fmaf(d.z, t, o.z), D, tnear, best, face);
   }
   if (d.y != 0.0f) {
-    const float ty = ((d.y > 0.0f ? D.ymax : D.ymin) - o.y) / d.y;
-    const float x = fmaf(d.x, ty, o.x), z = fmaf(d.z, ty, o.z);
-    if (ty >= tnear && ty <= best && x >= D.xmin && x <= D.xmax && z >= D.zmin && z <= D.zmax) {
-      best = ty;
-      face = d.y > 0.0f ? 3 : 1;
-    }
+    float t = (D.ymin - o.y) / d.y;
+    face_test(1, t, fmaf(d.x, t, o.x), D.xmin, D.xmax, fmaf(d.z, t, o.z), D, tnear, best, face);
+    t = (D.ymax - o.y) / d.y;
+    face_test(3, t, fmaf(d.x, t, o.x), D.xmin, D.xmax, fmaf(d.z, t, o.z), D, tnear, best, face);
   }
   tb = best;
   return face;
@@ -427,14 +437,41 @@ __device__ __forceinline__ uint32_t find_instance(const tparams& P, uint64_t gid
 }
 
 // Primary ray: rectangle_origin_z::get (src/rti/ray/rectangle_origin_z.hpp:23-37)
-// and power_cosine_direction_z::get (src/rti/ray/power_cosine_direction_z.hpp:17-32),
+// or disc_origin_z::get (src/rti/ray/disc_origin_z.hpp:17-33), and
+// power_cosine_direction_z::get (src/rti/ray/power_cosine_direction_z.hpp:17-32),
 // cosine_direction_z for n = 1 (src/rti/ray/cosine_direction_z.hpp:13-27).
+// Source disc of create_circular_source_from_bounding_box (src/rti/device.hpp:269-280):
+// centre of the box, radius half its x extent. Offsets uniform in [-r, r]^2,
+// redrawn until inside the disc; attempt a > 0 draws Philox block
+// (id, 0, a), which no segment draw uses (those have word 2 >= 1, word 3 = 0).
+// After kMaxSourceAttempts rejections (probability (1 - pi/4)^64 < 1e-42)
+// the ray starts at the centre, so the loop is bounded.
+constexpr uint32_t kMaxSourceAttempts = 64;
+
+__device__ __forceinline__ f3 disc_origin(const inst_desc& D, uint64_t id, uint32_t ux, uint32_t uy) {
+  const float cx = (D.xmin + D.xmax) * 0.5f, cy = (D.ymin + D.ymax) * 0.5f;
+  const float rad = (D.xmax - D.xmin) * 0.5f;
+  for (uint32_t a = 1;; ++a) {
+    const float dx = fmaf(2.0f, u01(ux), -1.0f) * rad;
+    const float dy = fmaf(2.0f, u01(uy), -1.0f) * rad;
+    if (sqrtf(fmaf(dx, dx, dy * dy)) <= rad) return make(cx + dx, cy + dy, D.src_z);
+    if (a == kMaxSourceAttempts) return make(cx, cy, D.src_z);
+    const u4 v = philox({(uint32_t)id, (uint32_t)(id >> 32), 0u, a}, D.seed, D.key1);
+    ux = v.x;
+    uy = v.y;
+  }
+}
+
 __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_state& R) {
   R.inst = find_instance(P, gid);
   const inst_desc& D = P.desc[R.inst];
   R.id = gid - D.ray_begin;
   const u4 u = philox({(uint32_t)R.id, (uint32_t)(R.id >> 32), 0u, 0u}, D.seed, D.key1);
-  R.o = make(fmaf(D.xmax - D.xmin, u01(u.x), D.xmin), fmaf(D.ymax - D.ymin, u01(u.y), D.ymin), D.src_z);
+  if (D.source == 0) {
+    R.o = make(fmaf(D.xmax - D.xmin, u01(u.x), D.xmin), fmaf(D.ymax - D.ymin, u01(u.y), D.ymin), D.src_z);
+  } else {
+    R.o = disc_origin(D, R.id, u.x, u.y);
+  }
   const float r1 = u01(u.z), r2 = u01_open0(u.w);
   const float tt = D.cos_exponent == 1.0f ? r2 : powf(r2, D.pow_e);
   float s, c;

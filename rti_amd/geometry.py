@@ -225,6 +225,18 @@ def flat_plate_discs(size=1.0, spacing=0.02, z=0.0):
     return _finish_discs(xyz, nrm, spacing * math.sqrt(2.0) / 2.0, spacing=spacing)
 
 
+def flat_plate_triangles(size=1.0, cells=32, z=0.0):
+    """Flat plate of cells x cells squares, two triangles each, normals +z."""
+    g = np.linspace(0.0, size, cells + 1)
+    GX, GY = np.meshgrid(g, g)
+    verts = np.stack([GX.ravel(), GY.ravel(), np.full(GX.size, z)], 1)
+    i, j = np.meshgrid(np.arange(cells), np.arange(cells))
+    v00 = (j * (cells + 1) + i).ravel()
+    v10, v01, v11 = v00 + 1, v00 + cells + 1, v00 + cells + 2
+    tris = np.concatenate([np.stack([v00, v10, v01], 1), np.stack([v10, v11, v01], 1)])
+    return {"kind": "triangles", "verts": verts.astype(np.float32), "tris": tris.astype(np.uint32)}
+
+
 def scene_box(scene, src_z=None):
     """Boundary box and source height the library derives from a scene.
 

@@ -25,12 +25,13 @@ numeric_type particle_t::sticking = 0.1f;
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: %s out.txt [rays] [sticking] [n]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s out.txt [rays] [sticking] [n] [circular 0|1]\n", argv[0]);
     return 2;
   }
   size_t rays = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 100000;
   particle_t::sticking = argc > 3 ? (numeric_type)std::atof(argv[3]) : 0.1f;
   numeric_type nexp = argc > 4 ? (numeric_type)std::atof(argv[4]) : 1.0f;
+  const bool circular = argc > 5 && std::atoi(argv[5]) != 0;
   // trench: width 1, depth 2, flats 0.5, y in [0, 0.5], 0.05 grid (same
   // sampling as rti_amd.geometry.trench_discs(depth=2, flat=0.5, ylen=0.5, spacing=0.05))
   const double h = 0.05, w2 = 0.5, depth = 2.0, flat = 0.5;
@@ -71,6 +72,7 @@ int main(int argc, char** argv) {
   dev.set_y(rti::geo::bound_condition::PERIODIC);
   auto dir = rti::ray::power_cosine_direction_z<numeric_type>(nexp);
   dev.set(dir);
+  dev.set_circular_source(circular);
   dev.run();
   auto est = dev.get_mc_estimates();
   auto cnt = dev.get_hit_cnts();

@@ -22,21 +22,24 @@ def _say(msg):
 # ---------------------------------------------------------------------------
 # drop-in and errors
 # ---------------------------------------------------------------------------
-def test_cpp_dropin_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("circular", [0, 1])
+def test_cpp_dropin_matches_c_abi(tmp_path, circular):
     """rti::device (include/rti/device.hpp) from C++ gives the same estimates
-    and hit counts, bit for bit, as the same run through the C ABI."""
+    and hit counts, bit for bit, as the same run through the C ABI (with the
+    rectangle source of device::run, and with the circular-source extension)."""
     exe = os.path.join(ROOT, "tests", "cpp", "dropin_trench")
     assert os.path.exists(exe), "built by __graft_entry__.build()"
     out = tmp_path / "est.txt"
-    _say("drop-in C++ example")
-    r = subprocess.run([exe, str(out), "100000", "0.1", "5"], capture_output=True, text=True, timeout=120)
+    _say(f"drop-in C++ example, circular source {circular}")
+    r = subprocess.run([exe, str(out), "100000", "0.1", "5", str(circular)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     data = np.loadtxt(out, dtype=np.float64)
     xyzr = np.concatenate([data[:, 0:3], data[:, 6:7]], axis=1).astype(np.float32)
     nrm = data[:, 3:6].astype(np.float32)
     with Context() as ctx:
         ctx.set_discs(xyzr, nrm)
-        ctx.set_params(n_rays=100000, sticking=0.1, bc_x=0, bc_y=1, cos_exponent=5.0)
+        ctx.set_params(n_rays=100000, sticking=0.1, bc_x=0, bc_y=1, cos_exponent=5.0, source=circular)
         ctx.run()
         g = ctx.flux()
         cnt = ctx.counts()

@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from rti_amd import Context, geometry, run_scene
-from tests.helpers import check_z, oracle_run, small_hole_mesh, small_trench, zscores
+from tests.helpers import check_z, oracle_run, small_hole_mesh, small_trench, tie_rank, zscores
 
 pytestmark = pytest.mark.gpu
 
@@ -46,18 +46,20 @@ def test_trench_common_random_numbers(s):
     """Same Philox streams: GPU and oracle follow the same paths until an
     FP32 rounding difference (GPU sincospi / FMA vs host libm) forks a
     reflected path, so the per-element sums agree far inside the MC error and
-    the hit counts agree exactly for most elements (all of them for s = 1,
-    where paths do not reflect)."""
+    the hit counts agree exactly for most elements (nearly all of them for
+    s = 1, where paths do not reflect). Hits at exactly equal distance
+    (coplanar overlapping discs) go to the lower stored position; the oracle
+    is given that order."""
     sc = small_trench()
     n = 200_000
     g = _gpu(sc, n, s, bc_x=0, bc_y=1)
-    o = oracle_run(sc, n, s, bc_x=0, bc_y=1)
+    o = oracle_run(sc, n, s, bc_x=0, bc_y=1, tie_rank=tie_rank(g["perm"]))
     assert g["stats"]["segments"] > n
     scale = np.maximum(np.sqrt(g["sigma"] ** 2 + o["s2"] - o["s1"] ** 2 / n), 1e-12)
     d = np.abs(g["raw"] - o["s1"]) / scale
     assert np.quantile(d, 0.99) < 0.5, np.quantile(d, [0.5, 0.9, 0.99, 1.0])
     same = np.mean(g["counts"] == o["cnt"])
-    assert same > (0.99 if s == 1.0 else 0.8), same
+    assert same > (0.995 if s == 1.0 else 0.9), same
     rel_seg = abs(g["stats"]["segments"] - o["segments"]) / o["segments"]
     assert rel_seg < 1e-3, (g["stats"]["segments"], o["segments"])
 

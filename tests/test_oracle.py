@@ -15,6 +15,7 @@ import pytest
 
 import oracle
 from rti_amd import geometry
+from tests.helpers import footprint_check
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -331,3 +332,75 @@ def test_error_stats_undefined_cases():
     # no deposits at all: every relative error is undefined
     rel0, _ = oracle.error_stats(np.zeros(2), np.zeros(2), np.zeros(2, dtype=np.uint64), total=0)
     assert np.all(rel0 == big)
+
+
+def test_disc_source_origins_uniform_in_inscribed_disc():
+    """disc_origin_z (src/rti/ray/disc_origin_z.hpp:17-33) on the disc of
+    create_circular_source_from_bounding_box (src/rti/device.hpp:269-280):
+    centre of the box, radius half its x extent. Uniform over the disc means
+    r^2 / R^2 ~ U(0, 1) and the angle ~ U(0, 2 pi) (a radius drawn uniformly,
+    the usual mistake, gives E[r^2] = R^2 / 3); the source plane is exact."""
+    from scipy import stats
+    box = (-1.5, 0.5, 2.0, 4.0, 0.0, 1.0)  # centre (-0.5, 3), R = 1
+    o = oracle.source_origins(box, 2.5, 200_000, source=1, seed=7, instance=3).astype(np.float64)
+    dx, dy = o[:, 0] + 0.5, o[:, 1] - 3.0
+    r2 = dx * dx + dy * dy
+    assert r2.max() <= 1.0 + 1e-6
+    assert np.all(o[:, 2] == np.float32(2.5))
+    assert stats.kstest(r2, "uniform").pvalue > 1e-3
+    assert stats.kstest((np.arctan2(dy, dx) + np.pi) / (2 * np.pi), "uniform").pvalue > 1e-3
+    assert abs(r2.mean() - 0.5) < 4 * math.sqrt(1 / 12 / r2.size)
+    assert abs(dx.mean()) < 4 * 0.5 / math.sqrt(r2.size) and abs(dy.mean()) < 4 * 0.5 / math.sqrt(r2.size)
+
+
+def test_disc_source_first_draw_layout():
+    """A ray whose first Philox block (ray, 0, 0, 0) gives offsets inside the
+    disc starts there (the rectangle source uses the same two words); the
+    others are redrawn, so no origin equals a rejected candidate."""
+    box = (0.0, 2.0, 0.0, 2.0, 0.0, 1.0)
+    n = 4000
+    o = oracle.source_origins(box, 1.0, n, source=1, seed=5, instance=2)
+    inside = 0
+    for ray in range(n):
+        u = oracle.philox([ray, 0, 0, 0], [5, 2])
+        f = (u[:2] >> 8).astype(np.float64) * 2.0 ** -24
+        d = (2 * f - 1).astype(np.float32) * np.float32(1.0)
+        cand = np.float32(1.0) + d
+        if float(d[0]) ** 2 + float(d[1]) ** 2 < 1.0 - 1e-6:
+            inside += 1
+            np.testing.assert_array_equal(o[ray, :2], cand)
+        elif float(d[0]) ** 2 + float(d[1]) ** 2 > 1.0 + 1e-6:
+            assert not np.array_equal(o[ray, :2], cand)
+    assert abs(inside / n - math.pi / 4) < 4 * math.sqrt(0.25 * 0.75 / n)
+
+
+def test_trace_disc_source_footprint_on_flat_mesh():
+    sc = geometry.flat_plate_triangles(size=1.0, cells=32)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 0.01)
+    n = 200_000
+    r = oracle.trace(sc, n_rays=n, box=box, src_z=0.01, sticking=1.0, bc_x=1, bc_y=1, cos_exponent=1e4,
+                     source=1)
+    assert r["surface_hits"] == n
+    footprint_check(r["cnt"].astype(np.float64), sc, n, (0.5, 0.5), 0.5)
+
+
+def test_tie_rank_orders_equal_distance_hits():
+    """Coplanar overlapping discs are hit at exactly equal distance; the
+    closest is the one of lower rank (default: lower index). The identity
+    rank is the default; another order moves deposits between elements (the
+    neighbour deposits follow the winner's neighbour list) and leaves the
+    counts of segments and hits alone."""
+    sc = small_trench_scene()
+    box, sz = geometry.scene_box(sc)
+    n = sc["xyzr"].shape[0]
+    kw = dict(n_rays=60_000, box=box, src_z=sz, sticking=1.0, bc_x=0, bc_y=1, threads=1)
+    a = oracle.trace(sc, **kw)
+    b = oracle.trace(sc, tie_rank=np.arange(n, dtype=np.uint32), **kw)
+    c = oracle.trace(sc, tie_rank=np.arange(n, dtype=np.uint32)[::-1].copy(), **kw)
+    assert np.array_equal(a["cnt"], b["cnt"]) and np.array_equal(a["s1"], b["s1"])
+    assert not np.array_equal(a["cnt"], c["cnt"])
+    assert (a["segments"], a["surface_hits"]) == (c["segments"], c["surface_hits"])
+
+
+def small_trench_scene():
+    return geometry.trench_discs(width=1.0, depth=2.0, flat=0.5, ylen=0.5, spacing=0.05)
