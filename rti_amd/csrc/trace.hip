@@ -56,6 +56,7 @@ struct tparams {
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
+  uint32_t block;  // threads per block (the LDS stack's slot stride), uniform
 };
 
 struct counters {
@@ -122,7 +123,7 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
 template <int LS, bool COUNT>
 __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint32_t* lds,
                                           uint32_t* ovf, counters& C) {
-  const int stride = blockDim.x;
+  const uint32_t stride = P.block;
   const bvh_node* nd = P.nodes + T.cur;
   const float4 a = nd->xy0, b = nd->xy1, z = nd->z01;
   const int4 ch = nd->child;
@@ -153,7 +154,7 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
     const bool near0 = tmin0 <= tmin1;
     const int far = near0 ? ch.y : ch.x;
     T.cur = near0 ? ch.x : ch.y;
-    if (T.sp < LS) lds[T.sp * stride] = (uint32_t)far;
+    if (T.sp < LS) lds[__umul24((uint32_t)T.sp, stride)] = (uint32_t)far;
     else ovf[T.sp - LS] = (uint32_t)far;
     ++T.sp;
     return false;
@@ -168,7 +169,7 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
   }
   if (T.sp == 0) return true;
   --T.sp;
-  T.cur = (int)(T.sp < LS ? lds[T.sp * stride] : ovf[T.sp - LS]);
+  T.cur = (int)(T.sp < LS ? lds[__umul24((uint32_t)T.sp, stride)] : ovf[T.sp - LS]);
   return false;
 }
 
@@ -182,8 +183,11 @@ typedef __attribute__((address_space(3))) uint64_t lds_u64;
 // Stack slot j of this thread, typed as LDS so that a conditional access to
 // either the LDS part or the private part cannot be merged into one generic
 // (flat) access.
-__device__ __forceinline__ lds_u64* lds_slot(uint32_t j) {
-  return (lds_u64*)(trace_lds) + j * blockDim.x + threadIdx.x;
+// The stride is the launch's block size passed as a kernel argument (a
+// uniform register; blockDim.x would be re-read from the dispatch packet with
+// a vector load at every access), j * stride fits the 24-bit multiply.
+__device__ __forceinline__ lds_u64* lds_slot(uint32_t j, uint32_t stride) {
+  return (lds_u64*)(trace_lds) + (__umul24(j, stride) + threadIdx.x);
 }
 
 __device__ __forceinline__ uint32_t sel4(int4 v, uint32_t c) {
@@ -275,7 +279,7 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
       if (ks[i] == 0xffffffffu) continue;
       // entry: node in the low word, entry distance bits in the high word
       const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | sel4(ch, ks[i] & 3u);
-      if (T.sp < LS) *lds_slot((uint32_t)T.sp) = e;
+      if (T.sp < LS) *lds_slot((uint32_t)T.sp, P.block) = e;
       else ovf[T.sp - LS] = e;
       ++T.sp;
     }
@@ -284,11 +288,15 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
   }
   while (T.sp > 0) {
     --T.sp;
-    // the private read is volatile so that the two reads are not merged into
-    // one generic (flat) access through a selected pointer
+    // the empty asm on the private read keeps the two reads from being merged
+    // into one generic (flat) access through a selected pointer
     uint64_t e;
-    if (T.sp < LS) e = *lds_slot((uint32_t)T.sp);
-    else e = const_cast<const volatile uint64_t*>(ovf)[T.sp - LS];
+    if (T.sp < LS) {
+      e = *lds_slot((uint32_t)T.sp, P.block);
+    } else {
+      e = ovf[T.sp - LS];
+      __asm__ volatile("" : "+v"(e));
+    }
     if (__uint_as_float((uint32_t)(e >> 32)) <= T.tbest) {
       T.cur = (int)(uint32_t)e;
       return false;
@@ -694,7 +702,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   counters C;
   ray_state R;
   trav_state T;
-  const unsigned long long gid = P.ray_lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long gid = P.ray_lo + (unsigned long long)blockIdx.x * P.block + threadIdx.x;
   if (gid < P.ray_hi) {
     start_ray(P, gid, R);
     for (;;) {
@@ -771,6 +779,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   const bool count = L.count != 0;
   const int bs = L.block_size;
   if (bs < 64 || bs > kMaxBlock || (bs & 63)) return hipErrorInvalidValue;
+  P.block = (uint32_t)bs;
   const kfn f = pick_kernel(L.width, L.lds_stack, persistent, L.adjoint, L.accum_fp64, count);
   if (!f) return hipErrorInvalidValue;
   const size_t shmem = stack_lds_bytes(L.width, bs, L.lds_stack);
