@@ -109,6 +109,11 @@ typedef struct rti_params {
   int32_t moments;        /* 1 = also accumulate sums of deposit^3 and deposit^4 (FP64)
                              for the variance of the variance                          */
   int32_t source;         /* rti_source (default RTI_SOURCE_RECTANGLE)                  */
+  int32_t leaf_batch;     /* persistent kernel, four-wide tree: leaf tests are queued per
+                             lane and run in wave-wide leaf phases once this many 64ths
+                             of the wave's tracing lanes hold a queued leaf (1..64);
+                             0 = default (64 when the store holds triangles, else -1);
+                             -1 = test leaves at each node visit                       */
 } rti_params;
 
 typedef struct rti_stats {

@@ -153,6 +153,7 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
     return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 8, 16, 24 or 32");
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
+  if (p.leaf_batch < -1 || p.leaf_batch > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..64");
   if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
     return fail(ctx, RTI_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
   if (p.moments != 0 && p.moments != 1) return fail(ctx, RTI_ERR_INVALID_ARG, "moments must be 0 or 1");
@@ -474,6 +475,7 @@ int upload_descriptors(rti_ctx* ctx) {
 
 struct kernel_cfg {
   int block, lds, grid, width;
+  int leaf;  // postponed-leaf threshold (64ths), 0 = leaf tests at each node
 };
 
 int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
@@ -481,6 +483,8 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
   cfg.block = p.block_size ? p.block_size : 256;
   cfg.width = p.bvh_width ? p.bvh_width : 4;
   cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width);
+  cfg.leaf = p.leaf_batch < 0 ? 0
+             : (p.leaf_batch ? p.leaf_batch : rti_impl::default_leaf_batch(ctx->prim_shift == rti_dev::kPrimShiftMixed));
   if (cfg.width == 4) {
     for (auto& I : ctx->inst)
       if ((int)I.wide_stack > rti_impl::wide_stack_capacity())
@@ -489,7 +493,7 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
                     rti_impl::wide_stack_capacity());
   }
   int per_cu = 0;
-  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, adj, fp64, cfg.width, &per_cu));
+  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, cfg.leaf, adj, fp64, cfg.width, &per_cu));
   if (per_cu < 1) per_cu = 1;
   if (p.waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, p.waves_per_cu * 64 / cfg.block));
   cfg.grid = per_cu * ctx->num_cus;
@@ -535,6 +539,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.count = count ? 1 : 0;
   L.compact = ctx->params.compact_lanes;
   L.width = cfg.width;
+  L.leaf_batch = cfg.leaf;
   if (kernel == RTI_KERNEL_PERSISTENT) {
     L.ray_lo = 0;
     L.ray_hi = ctx->rays_total;

@@ -99,14 +99,17 @@ struct trace_launch {
   int count;                        // 1 = instrumented run (node / prim / neighbour counters)
   int compact;                      // persistent: lanes done before a wave shades (1..64, 0 = default)
   int width;                        // traversal tree: 2 binary, 4 four-wide
+  int leaf_batch;                   // persistent four-wide: postponed leaf tests, a leaf phase once this many
+                                    // 64ths of the tracing lanes hold a queued leaf (1..64); 0 = at each node
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
-hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int width,
-                           int* blocks_per_cu);
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,
+                           int width, int* blocks_per_cu);
 int default_lds_stack(int width);
 // Stack entries (LDS + private) a four-wide traversal holds at most.
 int wide_stack_capacity();
 int default_compact_lanes();
+int default_leaf_batch(bool triangles);  // 0 = leaf tests at each node
 
 // ---- post ----
 hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);

@@ -56,7 +56,8 @@ struct tparams {
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
-  uint32_t block;  // threads per block (the LDS stack's slot stride), uniform
+  int leaf_min;  // postponed leaves: 64ths of the tracing lanes that must hold one
+  uint32_t block;  // threads per block (the LDS slot stride), uniform
 };
 
 struct counters {
@@ -73,8 +74,10 @@ struct trav_state {
   float tbest;      // starts at the boundary-face distance
   float tb;         // boundary-face distance
   int face;         // boundary face hit, -1 none
-  int cur;          // node being visited
+  int cur;          // node being visited (postponed leaves: -1 = no node left)
+  float curd;       // entry distance of cur (postponed leaves: skipped beyond tbest)
   int sp;           // stack depth
+  uint32_t pend;    // queued leaf primitives (postponed leaves)
   uint32_t pbest;   // closest primitive so far
   bool found;
 };
@@ -179,6 +182,7 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
 // stay ds_read / ds_write.
 extern __shared__ uint32_t trace_lds[];
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // Stack slot j of this thread, typed as LDS so that a conditional access to
 // either the LDS part or the private part cannot be merged into one generic
@@ -202,21 +206,23 @@ __device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
   return hit ? ((__float_as_uint(tmin) & ~3u) | c) : 0xffffffffu;
 }
 
-// One four-wide node per call. Leaf children are tested right away; hit
-// internal children are ordered by entry distance, the nearest is visited
-// next and the rest are pushed farthest first, each with its entry distance,
-// so that a popped entry beyond the closest hit found meanwhile is dropped
-// without fetching it. Returns true when the query is finished.
+// Child boxes of the four-wide node T.cur: entry distance and hit (child
+// present, slab interval non-empty within [tnear, tbest]) per child.
 //
 // Slab distances: the plane of byte q is org + q * 2^e, and
 //   t = (org + q * 2^e) * inv - o * inv = q * (2^e * inv) + (org * inv - o * inv),
 // one fma per plane with the two per-node factors.
-template <int LS, bool COUNT>
-__device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint64_t* ovf,
-                                           counters& C) {
+struct node4_hits {
+  float tmin[4];
+  bool hit[4];
+  int4 ch;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& T, node4_hits& H, counters& C) {
   const uint4* q = reinterpret_cast<const uint4*>(P.nodes4 + T.cur);
   const uint4 q0 = q[0], q1 = q[1], q2 = q[2];
-  const int4 ch = reinterpret_cast<const int4*>(q)[3];
+  H.ch = reinterpret_cast<const int4*>(q)[3];
   if (COUNT) C.nodes++;
   const f3 inv = T.inv, oi = T.oi;
   const uint32_t meta = q0.w;
@@ -231,8 +237,6 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
   const uint32_t nx = inv.x >= 0.0f ? q1.x : q1.w, fx = inv.x >= 0.0f ? q1.w : q1.x;
   const uint32_t ny = inv.y >= 0.0f ? q1.y : q2.x, fy = inv.y >= 0.0f ? q2.x : q1.y;
   const uint32_t nz = inv.z >= 0.0f ? q1.z : q2.y, fz = inv.z >= 0.0f ? q2.y : q1.z;
-  float tmin[4];
-  bool hit[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int sh = 8 * c;
@@ -242,24 +246,33 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
     const float tfx = fmaf((float)((fx >> sh) & 0xffu), ax, bx);
     const float tfy = fmaf((float)((fy >> sh) & 0xffu), ay, by);
     const float tfz = fmaf((float)((fz >> sh) & 0xffu), az, bz);
-    tmin[c] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, T.tnear));
+    H.tmin[c] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, T.tnear));
     const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, T.tbest));
-    hit[c] = (uint32_t)c < m && tmin[c] <= tmax;
+    H.hit[c] = (uint32_t)c < m && H.tmin[c] <= tmax;
   }
+}
+
+__device__ __forceinline__ uint32_t leaf_mask(const node4_hits& H) {
   uint32_t leaves = 0;
-  leaves |= (hit[0] && ch.x < 0) ? 1u : 0u;
-  leaves |= (hit[1] && ch.y < 0) ? 2u : 0u;
-  leaves |= (hit[2] && ch.z < 0) ? 4u : 0u;
-  leaves |= (hit[3] && ch.w < 0) ? 8u : 0u;
-  while (leaves) {
-    const uint32_t c = (uint32_t)(__ffs(leaves) - 1);
-    leaves &= leaves - 1;
-    test_prim<COUNT>(P, o, d, ~sel4(ch, c), T, C);
-  }
-  uint32_t k0 = sort_key(hit[0] && ch.x >= 0 && tmin[0] <= T.tbest, tmin[0], 0);
-  uint32_t k1 = sort_key(hit[1] && ch.y >= 0 && tmin[1] <= T.tbest, tmin[1], 1);
-  uint32_t k2 = sort_key(hit[2] && ch.z >= 0 && tmin[2] <= T.tbest, tmin[2], 2);
-  uint32_t k3 = sort_key(hit[3] && ch.w >= 0 && tmin[3] <= T.tbest, tmin[3], 3);
+  leaves |= (H.hit[0] && H.ch.x < 0) ? 1u : 0u;
+  leaves |= (H.hit[1] && H.ch.y < 0) ? 2u : 0u;
+  leaves |= (H.hit[2] && H.ch.z < 0) ? 4u : 0u;
+  leaves |= (H.hit[3] && H.ch.w < 0) ? 8u : 0u;
+  return leaves;
+}
+
+// Hit internal children ordered by entry distance: the nearest becomes
+// T.cur (with its distance in T.curd), the rest are pushed farthest first,
+// each with its entry distance, so that a popped entry beyond the closest hit
+// found meanwhile is dropped without fetching it. Children entered beyond
+// tbest are not kept. Returns false when no internal child is kept.
+template <int LS>
+__device__ __forceinline__ bool order_push(const tparams& P, trav_state& T, const node4_hits& H, uint64_t* ovf) {
+  const int4 ch = H.ch;
+  uint32_t k0 = sort_key(H.hit[0] && ch.x >= 0 && H.tmin[0] <= T.tbest, H.tmin[0], 0);
+  uint32_t k1 = sort_key(H.hit[1] && ch.y >= 0 && H.tmin[1] <= T.tbest, H.tmin[1], 1);
+  uint32_t k2 = sort_key(H.hit[2] && ch.z >= 0 && H.tmin[2] <= T.tbest, H.tmin[2], 2);
+  uint32_t k3 = sort_key(H.hit[3] && ch.w >= 0 && H.tmin[3] <= T.tbest, H.tmin[3], 3);
   {
     uint32_t a = min(k0, k1), b = max(k0, k1);
     k0 = a; k1 = b;
@@ -272,20 +285,26 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
     a = min(k1, k2); b = max(k1, k2);
     k1 = a; k2 = b;
   }
-  if (k0 != 0xffffffffu) {
-    const uint32_t ks[3] = {k3, k2, k1};
+  if (k0 == 0xffffffffu) return false;
+  const uint32_t ks[3] = {k3, k2, k1};
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      if (ks[i] == 0xffffffffu) continue;
-      // entry: node in the low word, entry distance bits in the high word
-      const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | sel4(ch, ks[i] & 3u);
-      if (T.sp < LS) *lds_slot((uint32_t)T.sp, P.block) = e;
-      else ovf[T.sp - LS] = e;
-      ++T.sp;
-    }
-    T.cur = (int)sel4(ch, k0 & 3u);
-    return false;
+  for (int i = 0; i < 3; ++i) {
+    if (ks[i] == 0xffffffffu) continue;
+    // entry: node in the low word, entry distance bits in the high word
+    const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | sel4(ch, ks[i] & 3u);
+    if (T.sp < LS) *lds_slot((uint32_t)T.sp, P.block) = e;
+    else ovf[T.sp - LS] = e;
+    ++T.sp;
   }
+  T.cur = (int)sel4(ch, k0 & 3u);
+  T.curd = __uint_as_float(k0 & ~3u);
+  return true;
+}
+
+// Pops entries until one lies within tbest (it becomes T.cur); false when
+// the stack runs empty.
+template <int LS>
+__device__ __forceinline__ bool pop4(const tparams& P, trav_state& T, uint64_t* ovf) {
   while (T.sp > 0) {
     --T.sp;
     // the empty asm on the private read keeps the two reads from being merged
@@ -297,12 +316,74 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
       e = ovf[T.sp - LS];
       __asm__ volatile("" : "+v"(e));
     }
-    if (__uint_as_float((uint32_t)(e >> 32)) <= T.tbest) {
+    const float dist = __uint_as_float((uint32_t)(e >> 32));
+    if (dist <= T.tbest) {
       T.cur = (int)(uint32_t)e;
-      return false;
+      T.curd = dist;
+      return true;
     }
   }
-  return true;
+  return false;
+}
+
+// One four-wide node per call, leaf children tested right away (before the
+// internal children are ordered, so they are culled with the updated closest
+// hit). Returns true when the query is finished.
+template <int LS, bool COUNT>
+__device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint64_t* ovf,
+                                           counters& C) {
+  node4_hits H;
+  node4_boxes<COUNT>(P, T, H, C);
+  uint32_t leaves = leaf_mask(H);
+  while (leaves) {
+    const uint32_t c = (uint32_t)(__ffs(leaves) - 1);
+    leaves &= leaves - 1;
+    test_prim<COUNT>(P, o, d, ~sel4(H.ch, c), T, C);
+  }
+  if (order_push<LS>(P, T, H, ovf)) return false;
+  return !pop4<LS>(P, T, ovf);
+}
+
+// Postponed leaf tests. A wave tests leaves only in leaf phases, where every
+// lane with a queued leaf tests one, instead of after each node visit, where
+// a node visit's leaves keep the whole wave while a few lanes test them.
+// Queued leaf j of thread t: the LDS word after the lane stacks at
+// j * block + t. A node visit queues at most four leaves.
+constexpr uint32_t kLeafQueue = 8;
+
+template <int LS>
+__device__ __forceinline__ lds_u32* leaf_slot(uint32_t j, uint32_t stride) {
+  return (lds_u32*)(trace_lds) + (2u * LS * stride + __umul24(j, stride) + threadIdx.x);
+}
+
+// Node phase: visits T.cur unless its entry distance lies beyond the closest
+// hit found since it was chosen, queues its hit leaf children, descends into
+// the nearest hit internal child or pops; T.cur = -1 when no node is left.
+template <int LS, bool COUNT>
+__device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, counters& C) {
+  if (T.curd <= T.tbest) {
+    node4_hits H;
+    node4_boxes<COUNT>(P, T, H, C);
+    const uint32_t leaves = leaf_mask(H);
+    uint32_t n = T.pend;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (leaves & (1u << c)) {
+        *leaf_slot<LS>(n, P.block) = ~sel4(H.ch, (uint32_t)c);
+        ++n;
+      }
+    }
+    T.pend = n;
+    if (order_push<LS>(P, T, H, ovf)) return;
+  }
+  if (!pop4<LS>(P, T, ovf)) T.cur = -1;
+}
+
+// Leaf phase: the most recently queued leaf of this lane.
+template <int LS, bool COUNT>
+__device__ __forceinline__ void leaf_visit(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
+  --T.pend;
+  test_prim<COUNT>(P, o, d, *leaf_slot<LS>(T.pend, P.block), T, C);
 }
 
 // Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
@@ -325,6 +406,10 @@ struct lane_stack<LS, 4> {
   template <bool COUNT>
   __device__ __forceinline__ bool step(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
     return trav4_step<LS, COUNT>(P, o, d, T, ovf, C);
+  }
+  template <bool COUNT>
+  __device__ __forceinline__ void visit(const tparams& P, trav_state& T, counters& C) {
+    node4_visit<LS, COUNT>(P, T, ovf, C);
   }
 };
 
@@ -508,7 +593,9 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.found = false;
   T.pbest = 0;
   T.cur = W == 4 ? D.root4 : D.root;
+  T.curd = 0.0f;
   T.sp = 0;
+  T.pend = 0;
   const float tiny = 1e-20f;
   // slab factors only cull boxes (every box is padded beyond its
   // primitives), so the hardware reciprocal (1 ulp) suffices
@@ -642,7 +729,15 @@ __device__ __forceinline__ void flush_counters(const tparams& P, const counters&
 // spills nothing; the adjoint variant carries four more live values per ray
 // and gets 6 (80 VGPRs). The launch can still run fewer waves per CU
 // (waves_per_cu) where the memory system prefers it.
-template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT>
+//
+// With postponed leaves (PL, four-wide tree) each traversal iteration is
+// either a node phase (lanes with a node left visit it and queue its hit
+// leaves) or a leaf phase (lanes with a queued leaf test one), chosen for
+// the whole wave: a leaf phase runs once at least leaf_min / 64 of the
+// tracing lanes hold a queued leaf, when a lane's queue could not take
+// another node's four, or when no node is left. A lane's query is open
+// while it has a node or a queued leaf left.
+template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT, bool PL>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? 6 : 7))) void k_trace_persistent(
     tparams P) {
   lane_stack<LS, W> S;
@@ -680,9 +775,27 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     // wave drains (queue empty, fewer live lanes) the bound shrinks with it,
     // so a few long paths still advance many nodes per shading pass.
     const int still = ((64 - compact) * __popcll(live)) >> 6;
-    for (;;) {
-      if (tracing) tracing = !S.template step<COUNT>(P, R.o, R.d, T, C);
-      if (__popcll(__ballot(tracing)) <= still) break;
+    if constexpr (PL) {
+      for (;;) {
+        const unsigned long long tr = __ballot(tracing);
+        const unsigned long long queued = __ballot(tracing && T.pend > 0);
+        const unsigned long long full = __ballot(tracing && T.pend > kLeafQueue - 4);
+        const unsigned long long nodes = __ballot(tracing && T.cur >= 0);
+        const bool leaf_phase =
+            queued != 0ull && (full != 0ull || nodes == 0ull || __popcll(queued) * 64 >= P.leaf_min * __popcll(tr));
+        if (leaf_phase) {
+          if (tracing && T.pend > 0) leaf_visit<LS, COUNT>(P, R.o, R.d, T, C);
+        } else {
+          if (tracing && T.cur >= 0) S.template visit<COUNT>(P, T, C);
+        }
+        tracing = tracing && (T.cur >= 0 || T.pend > 0);
+        if (__popcll(__ballot(tracing)) <= still) break;
+      }
+    } else {
+      for (;;) {
+        if (tracing) tracing = !S.template step<COUNT>(P, R.o, R.d, T, C);
+        if (__popcll(__ballot(tracing)) <= still) break;
+      }
     }
     if (alive && !tracing) {
       alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
@@ -717,44 +830,59 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
 
 using kfn = void (*)(tparams);
 
+template <int LS, int W, bool PL>
+kfn pick_persistent(bool adj, bool fp64, bool count) {
+  if (count) return k_trace_persistent<LS, W, false, false, true, PL>;
+  if (adj) return fp64 ? k_trace_persistent<LS, W, true, true, false, PL> : k_trace_persistent<LS, W, true, false, false, PL>;
+  return fp64 ? k_trace_persistent<LS, W, false, true, false, PL> : k_trace_persistent<LS, W, false, false, false, PL>;
+}
+
 template <int LS, int W>
-kfn pick(bool persistent, bool adj, bool fp64, bool count) {
-  if (count)
-    return persistent ? k_trace_persistent<LS, W, false, false, true> : k_trace_plain<LS, W, false, false, true>;
+kfn pick(bool persistent, bool postpone, bool adj, bool fp64, bool count) {
   if (persistent) {
-    if (adj) return fp64 ? k_trace_persistent<LS, W, true, true, false> : k_trace_persistent<LS, W, true, false, false>;
-    return fp64 ? k_trace_persistent<LS, W, false, true, false> : k_trace_persistent<LS, W, false, false, false>;
+    if constexpr (W == 4) {
+      if (postpone) return pick_persistent<LS, W, true>(adj, fp64, count);
+    }
+    return pick_persistent<LS, W, false>(adj, fp64, count);
   }
+  if (count) return k_trace_plain<LS, W, false, false, true>;
   if (adj) return fp64 ? k_trace_plain<LS, W, true, true, false> : k_trace_plain<LS, W, true, false, false>;
   return fp64 ? k_trace_plain<LS, W, false, true, false> : k_trace_plain<LS, W, false, false, false>;
 }
 
 template <int W>
-kfn pick_ls(int lds_stack, bool persistent, bool adj, bool fp64, bool count) {
+kfn pick_ls(int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count) {
   switch (lds_stack) {
-    case 4: return pick<4, W>(persistent, adj, fp64, count);
-    case 8: return pick<8, W>(persistent, adj, fp64, count);
-    case 16: return pick<16, W>(persistent, adj, fp64, count);
-    case 24: return pick<24, W>(persistent, adj, fp64, count);
-    case 32: return pick<32, W>(persistent, adj, fp64, count);
+    case 4: return pick<4, W>(persistent, postpone, adj, fp64, count);
+    case 8: return pick<8, W>(persistent, postpone, adj, fp64, count);
+    case 16: return pick<16, W>(persistent, postpone, adj, fp64, count);
+    case 24: return pick<24, W>(persistent, postpone, adj, fp64, count);
+    case 32: return pick<32, W>(persistent, postpone, adj, fp64, count);
     default: return nullptr;
   }
 }
 
-kfn pick_kernel(int width, int lds_stack, bool persistent, bool adj, bool fp64, bool count) {
-  if (width == 2) return pick_ls<2>(lds_stack, persistent, adj, fp64, count);
-  if (width == 4) return pick_ls<4>(lds_stack, persistent, adj, fp64, count);
+kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count) {
+  if (width == 2) return pick_ls<2>(lds_stack, persistent, false, adj, fp64, count);
+  if (width == 4) return pick_ls<4>(lds_stack, persistent, postpone, adj, fp64, count);
   return nullptr;
 }
 
-size_t stack_lds_bytes(int width, int block_size, int lds_stack) {
-  return (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
+// Dynamic LDS: the lane stacks, then (postponed leaves) the leaf queues.
+size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) {
+  size_t b = (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
+  if (width == 4 && postpone) b += (size_t)block_size * kLeafQueue * 4;
+  return b;
 }
 
 }  // namespace
 
 int default_lds_stack(int width) { return width == 4 ? 8 : 16; }
 int default_compact_lanes() { return 56; }
+// Postponed leaves pay off where a leaf test is expensive: triangle stores
+// (measured +5 % on a 199k-triangle hole) but not stores of discs only, whose
+// leaf test is a few instructions (batch of disc clouds -5 %, rough surface -3 %).
+int default_leaf_batch(bool triangles) { return triangles ? 64 : 0; }
 int wide_stack_capacity() { return kWideStack; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
@@ -775,14 +903,16 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.queue = L.queue;
   P.stats = L.stats;
   P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();
+  P.leaf_min = L.leaf_batch > 64 ? 64 : L.leaf_batch;
   const bool persistent = L.kernel == 0;
+  const bool postpone = persistent && L.width == 4 && L.leaf_batch > 0;
   const bool count = L.count != 0;
   const int bs = L.block_size;
   if (bs < 64 || bs > kMaxBlock || (bs & 63)) return hipErrorInvalidValue;
   P.block = (uint32_t)bs;
-  const kfn f = pick_kernel(L.width, L.lds_stack, persistent, L.adjoint, L.accum_fp64, count);
+  const kfn f = pick_kernel(L.width, L.lds_stack, persistent, postpone, L.adjoint, L.accum_fp64, count);
   if (!f) return hipErrorInvalidValue;
-  const size_t shmem = stack_lds_bytes(L.width, bs, L.lds_stack);
+  const size_t shmem = trace_lds_bytes(L.width, bs, L.lds_stack, postpone);
   uint64_t grid;
   if (persistent) {
     grid = (uint64_t)L.grid_blocks;
@@ -794,12 +924,13 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int adjoint, int fp64, int width,
-                           int* blocks_per_cu) {
-  const kfn f = pick_kernel(width, lds_stack, persistent, adjoint, fp64, false);
+hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,
+                           int width, int* blocks_per_cu) {
+  const bool postpone = persistent && width == 4 && leaf_batch > 0;
+  const kfn f = pick_kernel(width, lds_stack, persistent, postpone, adjoint, fp64, false);
   if (!f) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block_size,
-                                                      stack_lds_bytes(width, block_size, lds_stack));
+                                                      trace_lds_bytes(width, block_size, lds_stack, postpone));
 }
 
 }  // namespace rti_impl

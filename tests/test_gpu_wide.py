@@ -131,7 +131,13 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     a = run_scene(sc, bvh_width=4, **base)
     b = run_scene(sc, bvh_width=2, **base)
     c = run_scene(sc, bvh_width=4, kernel=1, **base)
-    for other in (b, c):
+    # leaf tests at each node visit, and postponed leaf tests with a leaf
+    # phase as soon as one lane queues a leaf / only when every tracing lane
+    # holds one (small block and LDS stack: the private stack part is used)
+    d = run_scene(sc, bvh_width=4, leaf_batch=-1, **base)
+    e = run_scene(sc, bvh_width=4, leaf_batch=1, **base)
+    f = run_scene(sc, bvh_width=4, leaf_batch=64, block_size=64, lds_stack=4, **base)
+    for other in (b, c, d, e, f):
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["raw"], other["raw"])
         assert a["stats"]["segments"] == other["stats"]["segments"]
@@ -151,3 +157,10 @@ def test_invalid_width_is_rejected():
     with Context() as ctx:
         with pytest.raises(Exception):
             ctx.set_params(bvh_width=3)
+
+
+def test_invalid_leaf_batch_is_rejected():
+    with Context() as ctx:
+        for v in (-2, 65):
+            with pytest.raises(Exception):
+                ctx.set_params(leaf_batch=v)
