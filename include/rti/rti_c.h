@@ -84,6 +84,13 @@ enum rti_accum { RTI_ACCUM_FIXED64 = 0, RTI_ACCUM_FP64 = 1 };
  * device::create_circular_source_from_bounding_box, device.hpp:269-280). */
 enum rti_source { RTI_SOURCE_RECTANGLE = 0, RTI_SOURCE_DISC = 1 };
 
+/* Topology of the binary tree the traversal trees are built from (both over
+ * the primitives' Morton order, with a bottom-up box refit; replaces the
+ * Embree BVH build of tracer.hpp:74-100). LBVH: the Karras radix tree of the
+ * Morton codes. PLOC: locally-ordered clustering (mutual nearest neighbours
+ * by merged surface area within 16 positions), fewer node visits per ray. */
+enum rti_tree_builder { RTI_TREE_LBVH = 1, RTI_TREE_PLOC = 2 };
+
 typedef struct rti_params {
   uint64_t n_rays;        /* primary rays (reference default 1024, device.hpp:289)    */
   uint32_t seed;          /* Philox key word 0                                        */
@@ -114,6 +121,8 @@ typedef struct rti_params {
                              of the wave's tracing lanes hold a queued leaf (1..64);
                              0 = default (64 when the store holds triangles, else -1);
                              -1 = test leaves at each node visit                       */
+  int32_t tree_builder;   /* rti_tree_builder; 0 = default (PLOC for triangle meshes,
+                             LBVH for disc clouds)                                      */
 } rti_params;
 
 typedef struct rti_stats {

@@ -154,6 +154,8 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
   if (p.leaf_batch < -1 || p.leaf_batch > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..64");
+  if (p.tree_builder != 0 && p.tree_builder != RTI_TREE_LBVH && p.tree_builder != RTI_TREE_PLOC)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "tree_builder must be 0, RTI_TREE_LBVH or RTI_TREE_PLOC");
   if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
     return fail(ctx, RTI_ERR_INVALID_ARG, "bvh_width must be 0, 2 or 4");
   if (p.moments != 0 && p.moments != 1) return fail(ctx, RTI_ERR_INVALID_ARG, "moments must be 0 or 1");
@@ -287,6 +289,16 @@ __global__ void k_set_u32(uint32_t* p, uint32_t n, uint32_t v) {
 
 __global__ void k_set_one_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// Binary tree topology. Default: PLOC for triangle meshes (measured +1.5 %
+// on a 199k-triangle hole), the Karras tree for disc clouds (PLOC measured
+// -1 % on a batch of disc trenches and holes, -4 % on a 4.2M-disc rough
+// surface, whose regular grids Morton order already clusters well).
+int tree_builder(const rti_params& p, int kind) {
+  if (p.tree_builder == RTI_TREE_LBVH) return rti_impl::kBuildKarras;
+  if (p.tree_builder == RTI_TREE_PLOC) return rti_impl::kBuildPloc;
+  return kind == 1 ? rti_impl::kBuildPloc : rti_impl::kBuildKarras;
+}
+
 int prepare(rti_ctx* ctx) {
   if (ctx->inst.empty()) return fail(ctx, RTI_ERR_NO_GEOMETRY, "no geometry set");
   if (!ctx->dirty) return RTI_OK;
@@ -342,6 +354,7 @@ int prepare(rti_ctx* ctx) {
     J.area_box[1] = I.box[2];
     J.area_box[2] = I.box[1];
     J.area_box[3] = I.box[3];
+    J.builder = tree_builder(ctx->params, I.kind);
     if (I.kind == 0) {
       HIP_TRY(ctx, in_a.ensure(I.n * 16));
       HIP_TRY(ctx, in_b.ensure(I.n * 12));
@@ -835,6 +848,8 @@ int rti_set_params(rti_ctx* ctx, const rti_params* p) {
   rti_params q = *p;
   int rc = check_params(ctx, q);
   if (rc) return rc;
+  // another tree topology needs another build
+  if (q.tree_builder != ctx->params.tree_builder) ctx->dirty = true;
   ctx->params = q;
   if (!ctx->batch_mode)
     for (auto& I : ctx->inst) I.params = q;

@@ -358,6 +358,107 @@ __global__ void k_refit(int n, uint32_t prim_begin, uint32_t node_begin, bvh_nod
   }
 }
 
+// ---------------------------------------------------------------------------
+// PLOC (Meister and Bittner, "Parallel Locally-Ordered Clustering for
+// Bounding Volume Hierarchy Construction", IEEE TVCG 2018): the clusters
+// (initially the leaves in Morton order) each find the cluster within
+// kPlocRadius positions whose merged box has the smallest surface area;
+// mutual nearest neighbours merge into a new binary node, and the survivors
+// are compacted in order. Every iteration merges at least the pair that is
+// smallest in the total order (area, lower index, higher index), so the
+// loop ends after at most n - 1 iterations. The merges of an iteration with
+// c clusters and m merges take node slots c - m - 1 .. c - 2 (in index
+// order), so the last merge is node 0, the root; the node boxes are then
+// written by the same bottom-up refit as the Karras tree.
+// ---------------------------------------------------------------------------
+constexpr int kPlocRadius = 16;
+
+__global__ void k_ploc_init(uint32_t n, uint32_t prim_begin, const float4* leaf_lo, const float4* leaf_hi, int* cid,
+                            float4* clo, float4* chi) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cid[j] = ~(int)(prim_begin + j);
+  clo[j] = leaf_lo[j];
+  chi[j] = leaf_hi[j];
+}
+
+// Half the surface area of the union of two boxes; symmetric in its
+// arguments bit for bit (min / max commute), so both clusters of a pair
+// see the same value.
+__device__ __forceinline__ float merged_area(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+  const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+  const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Nearest neighbour of each cluster. Candidates are scanned in increasing
+// index, for which the pair keys (lower index, higher index) increase, so
+// keeping the first of equal areas realises the total order on pairs.
+__global__ void k_ploc_nn(uint32_t c, const float4* clo, const float4* chi, uint32_t* nn) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= (int)c) return;
+  const float4 alo = clo[i], ahi = chi[i];
+  const int lo = max(0, i - kPlocRadius), hi = min((int)c - 1, i + kPlocRadius);
+  float best = FLT_MAX;
+  int bj = i == lo ? hi : lo;
+  for (int j = lo; j <= hi; ++j) {
+    if (j == i) continue;
+    const float a = merged_area(alo, ahi, clo[j], chi[j]);
+    if (a < best) {
+      best = a;
+      bj = j;
+    }
+  }
+  nn[i] = (uint32_t)bj;
+}
+
+// keep[i]: cluster i survives (it is not the higher-index half of a mutual
+// pair); lead[i]: cluster i is the lower-index half of a mutual pair.
+__global__ void k_ploc_flags(uint32_t c, const uint32_t* nn, uint32_t* keep, uint32_t* lead) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    keep[c] = 0;
+    lead[c] = 0;
+  }
+  if (i >= c) return;
+  const uint32_t j = nn[i];
+  const bool mutual = nn[j] == i;
+  keep[i] = (mutual && j < i) ? 0u : 1u;
+  lead[i] = (mutual && i < j) ? 1u : 0u;
+}
+
+// keep_pos / lead_pos: exclusive scans of keep / lead over c + 1 entries
+// (entry c holds the totals).
+__global__ void k_ploc_merge(uint32_t c, uint32_t n, uint32_t prim_begin, uint32_t node_begin, const uint32_t* nn,
+                             const uint32_t* keep_pos, const uint32_t* lead_pos, const int* cid, const float4* clo,
+                             const float4* chi, int* cid_out, float4* clo_out, float4* chi_out, bvh_node* nodes,
+                             int32_t* parent) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c) return;
+  const uint32_t j = nn[i];
+  const bool mutual = nn[j] == i;
+  if (mutual && j < i) return;
+  const uint32_t p = keep_pos[i];
+  if (!mutual) {
+    cid_out[p] = cid[i];
+    clo_out[p] = clo[i];
+    chi_out[p] = chi[i];
+    return;
+  }
+  const uint32_t id = c - lead_pos[c] - 1 + lead_pos[i];
+  const int a = cid[i], b = cid[j];
+  nodes[node_begin + id].child = make_int4(a, b, 0, 0);
+  // parent[] local ids: internal nodes 0..n-2, leaves n-1 .. 2n-2
+  parent[a < 0 ? (n - 1) + ((uint32_t)~a - prim_begin) : (uint32_t)a - node_begin] = (int32_t)id;
+  parent[b < 0 ? (n - 1) + ((uint32_t)~b - prim_begin) : (uint32_t)b - node_begin] = (int32_t)id;
+  if (id == 0) parent[0] = -1;
+  const float4 alo = clo[i], ahi = chi[i], blo = clo[j], bhi = chi[j];
+  cid_out[p] = (int)(node_begin + id);
+  clo_out[p] = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), 0.0f);
+  chi_out[p] = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), 0.0f);
+}
+
 // One primitive: a root whose two children are that leaf.
 __global__ void k_single_root(uint32_t prim_begin, uint32_t node_begin, bvh_node* nodes, const float4* leaf_lo,
                               const float4* leaf_hi, uint32_t* max_depth) {
@@ -757,17 +858,57 @@ hipError_t scratch_reserve(build_scratch& s, uint32_t n) {
   RTI_ALLOC(s.leaf_hi, cap * 16ull);
   RTI_ALLOC(s.counts, (cap + 1ull) * 4ull);
   RTI_ALLOC(s.depth, 4ull);
+  RTI_ALLOC(s.clo[0], cap * 16ull);
+  RTI_ALLOC(s.clo[1], cap * 16ull);
+  RTI_ALLOC(s.chi[0], cap * 16ull);
+  RTI_ALLOC(s.chi[1], cap * 16ull);
+  RTI_ALLOC(s.lead, (cap + 1ull) * 4ull);
 #undef RTI_ALLOC
   s.cap = cap;
   return hipSuccess;
 }
 
 void scratch_free(build_scratch& s) {
-  void* ptrs[] = {s.keys[0], s.keys[1], s.vals[0], s.vals[1], s.hist, s.scan_tmp, s.parent,
-                  s.flags, s.leaf_lo, s.leaf_hi, s.counts, s.depth};
+  void* ptrs[] = {s.keys[0], s.keys[1], s.vals[0], s.vals[1], s.hist, s.scan_tmp, s.parent, s.flags, s.leaf_lo,
+                  s.leaf_hi, s.counts, s.depth, s.clo[0], s.clo[1], s.chi[0], s.chi[1], s.lead};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s = build_scratch{};
+}
+
+// PLOC topology (child ids and parent links) over the instance's leaves,
+// which k_scatter_prims has stored in Morton order with their boxes. The
+// sort buffers are free here: keys[] hold the cluster ids, vals[0] the
+// nearest neighbours, counts[] the survivor flags.
+hipError_t build_ploc(const build_job& job, const store_ptrs& st, build_scratch& s, hipStream_t stream) {
+  const uint32_t n = job.n;
+  const unsigned T = 256;
+  int* cid[2] = {reinterpret_cast<int*>(s.keys[0]), reinterpret_cast<int*>(s.keys[1])};
+  k_ploc_init<<<blocks_for(n, T), T, 0, stream>>>(n, job.prim_begin, s.leaf_lo, s.leaf_hi, cid[0], s.clo[0],
+                                                  s.chi[0]);
+  uint32_t c = n;
+  int cur = 0;
+  hipError_t e;
+  while (c > 1) {
+    k_ploc_nn<<<blocks_for(c, T), T, 0, stream>>>(c, s.clo[cur], s.chi[cur], s.vals[0]);
+    k_ploc_flags<<<blocks_for(c, T), T, 0, stream>>>(c, s.vals[0], s.counts, s.lead);
+    e = exclusive_scan_u32(s.counts, s.counts, c + 1, s.scan_tmp, stream);
+    if (e != hipSuccess) return e;
+    e = exclusive_scan_u32(s.lead, s.lead, c + 1, s.scan_tmp, stream);
+    if (e != hipSuccess) return e;
+    k_ploc_merge<<<blocks_for(c, T), T, 0, stream>>>(c, n, job.prim_begin, job.node_begin, s.vals[0], s.counts,
+                                                     s.lead, cid[cur], s.clo[cur], s.chi[cur], cid[cur ^ 1],
+                                                     s.clo[cur ^ 1], s.chi[cur ^ 1], st.nodes, s.parent);
+    uint32_t next = 0;
+    e = hipMemcpyAsync(&next, s.counts + c, 4, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    if (next >= c || next == 0) return hipErrorUnknown;  // no merge: cannot happen (see above)
+    c = next;
+    cur ^= 1;
+  }
+  return hipGetLastError();
 }
 
 hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* depth_out,
@@ -811,8 +952,13 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
   if (n == 1) {
     k_single_root<<<1, 1, 0, stream>>>(job.prim_begin, job.node_begin, st.nodes, s.leaf_lo, s.leaf_hi, s.depth);
   } else {
-    k_karras<<<blocks_for(n - 1, T), T, 0, stream>>>(s.keys[cur], (int)n, job.prim_begin, job.node_begin, st.nodes,
-                                                     s.parent);
+    if (job.builder == kBuildPloc) {
+      e = build_ploc(job, st, s, stream);
+      if (e != hipSuccess) return e;
+    } else {
+      k_karras<<<blocks_for(n - 1, T), T, 0, stream>>>(s.keys[cur], (int)n, job.prim_begin, job.node_begin,
+                                                       st.nodes, s.parent);
+    }
     e = hipMemsetAsync(s.flags, 0, (n - 1) * 4ull, stream);
     if (e != hipSuccess) return e;
     k_refit<<<blocks_for(n, T), T, 0, stream>>>((int)n, job.prim_begin, job.node_begin, st.nodes, s.parent, s.flags,

@@ -24,7 +24,10 @@ struct build_job {
   float cmin[3], cmax[3];   // centroid bounds (for Morton quantisation)
   float rmax;               // largest disc radius (neighbour query reach)
   float area_box[4];        // xmin, ymin, xmax, ymax for the exposed area
+  int builder;              // binary tree topology: kBuildKarras or kBuildPloc
 };
+constexpr int kBuildKarras = 1;  // Karras radix tree over the Morton codes
+constexpr int kBuildPloc = 2;    // locally-ordered clustering (PLOC) in Morton order
 
 // Scene store (absolute indexing; see rti_device.hpp).
 struct store_ptrs {
@@ -49,8 +52,11 @@ struct build_scratch {
   uint32_t* flags = nullptr;      // n - 1
   float4* leaf_lo = nullptr;      // n
   float4* leaf_hi = nullptr;      // n
-  uint32_t* counts = nullptr;     // n + 1 (neighbour degrees)
+  uint32_t* counts = nullptr;     // n + 1 (neighbour degrees; PLOC survivor flags)
   uint32_t* depth = nullptr;      // 1
+  float4* clo[2] = {nullptr, nullptr};  // n: PLOC cluster boxes (ping-pong)
+  float4* chi[2] = {nullptr, nullptr};
+  uint32_t* lead = nullptr;       // n + 1: PLOC merge flags
 };
 
 hipError_t scratch_reserve(build_scratch& s, uint32_t n);

@@ -63,30 +63,32 @@ def _validate(nodes, perm, scene):
     assert bad == 0, f"{bad} child boxes do not enclose their subtree"
 
 
+@pytest.mark.parametrize("builder", [1, 2], ids=["lbvh", "ploc"])
 @pytest.mark.parametrize("which", ["small_trench", "trench_fine", "hole_discs"])
-def test_hierarchy_encloses_every_primitive(which):
+def test_hierarchy_encloses_every_primitive(which, builder):
     sc = {"small_trench": small_trench(),
           "trench_fine": geometry.trench_discs(spacing=0.02, depth=1.0, flat=0.3, ylen=0.3),
           "hole_discs": geometry.hole_discs(spacing=0.05)}[which]
-    print(f"[gpu] hierarchy {which} n={sc['xyzr'].shape[0]}", flush=True)
+    print(f"[gpu] hierarchy {which} n={sc['xyzr'].shape[0]} tree_builder {builder}", flush=True)
     with Context() as ctx:
         ctx.set_scene(sc)
-        ctx.set_params(n_rays=16, sticking=1.0)
+        ctx.set_params(n_rays=16, sticking=1.0, tree_builder=builder)
         nodes, perm, pa = ctx.hierarchy()
         np.testing.assert_array_equal(pa, sc["xyzr"][perm])
         _validate(nodes, perm, sc)
 
 
-@pytest.mark.parametrize("which", ["small_trench", "trench_c1", "hole_discs", "rough"])
-def test_neighbor_graph_equals_oracle(which):
+@pytest.mark.parametrize("which,builder", [("small_trench", 0), ("trench_c1", 0), ("hole_discs", 0), ("rough", 0),
+                                           ("trench_c1", 2), ("hole_discs", 2)])
+def test_neighbor_graph_equals_oracle(which, builder):
     sc = {"small_trench": small_trench(),
           "trench_c1": geometry.trench_discs(),
           "hole_discs": geometry.hole_discs(spacing=0.04),
           "rough": geometry.rough_surface_discs(nfield=256, cells=1, spacing=0.02)}[which]
-    print(f"[gpu] neighbours {which} n={sc['xyzr'].shape[0]}", flush=True)
+    print(f"[gpu] neighbours {which} n={sc['xyzr'].shape[0]} tree_builder {builder}", flush=True)
     with Context() as ctx:
         ctx.set_scene(sc)
-        ctx.set_params(n_rays=16, sticking=1.0)
+        ctx.set_params(n_rays=16, sticking=1.0, tree_builder=builder)
         row, col = ctx.neighbors()
     orow, ocol = oracle.neighbors(sc["xyzr"])
     bad = np.nonzero(np.diff(row.astype(np.int64)) != np.diff(orow.astype(np.int64)))[0]

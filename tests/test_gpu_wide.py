@@ -101,14 +101,15 @@ SCENES = {
 }
 
 
+@pytest.mark.parametrize("builder", [1, 2], ids=["lbvh", "ploc"])
 @pytest.mark.parametrize("which", list(SCENES))
-def test_wide_tree_encloses_every_primitive(which):
+def test_wide_tree_encloses_every_primitive(which, builder):
     sc = SCENES[which]()
     n = sc["xyzr"].shape[0] if sc["kind"] == "discs" else sc["tris"].shape[0]
-    print(f"[gpu] wide tree {which} n={n}", flush=True)
+    print(f"[gpu] wide tree {which} n={n} tree_builder {builder}", flush=True)
     with Context() as ctx:
         ctx.set_scene(sc)
-        ctx.set_params(n_rays=16, sticking=1.0)
+        ctx.set_params(n_rays=16, sticking=1.0, tree_builder=builder)
         _, perm, _ = ctx.hierarchy()
         wide = ctx.wide_hierarchy()
         st = ctx.stats()
@@ -137,7 +138,10 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     d = run_scene(sc, bvh_width=4, leaf_batch=-1, **base)
     e = run_scene(sc, bvh_width=4, leaf_batch=1, **base)
     f = run_scene(sc, bvh_width=4, leaf_batch=64, block_size=64, lds_stack=4, **base)
-    for other in (b, c, d, e, f):
+    # the other binary-tree topology: the closest hit does not depend on the tree
+    g = run_scene(sc, tree_builder=1, **base)
+    h = run_scene(sc, tree_builder=2, **base)
+    for other in (b, c, d, e, f, g, h):
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["raw"], other["raw"])
         assert a["stats"]["segments"] == other["stats"]["segments"]
@@ -164,3 +168,10 @@ def test_invalid_leaf_batch_is_rejected():
         for v in (-2, 65):
             with pytest.raises(Exception):
                 ctx.set_params(leaf_batch=v)
+
+
+def test_invalid_tree_builder_is_rejected():
+    with Context() as ctx:
+        for v in (-1, 3):
+            with pytest.raises(Exception):
+                ctx.set_params(tree_builder=v)
