@@ -99,7 +99,7 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
     const f3 n = make(B.x, B.y, B.z);
     const float dn = dot(d, n);
     const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
-    th = dot(co, n) / dn;
+    th = dot(co, n) * __builtin_amdgcn_rcpf(dn);
     const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
     ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
   } else {
@@ -108,7 +108,7 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
     const f3 e1 = make(B.x, B.y, B.z), e2 = make(C2.x, C2.y, C2.z);
     const f3 pv = cross(d, e2);
     const float det = dot(e1, pv);
-    const float idet = 1.0f / det;
+    const float idet = __builtin_amdgcn_rcpf(det);
     const f3 tv = make(o.x - A.x, o.y - A.y, o.z - A.z);
     const float u = dot(tv, pv) * idet;
     const f3 qv = cross(tv, e1);
