@@ -102,6 +102,23 @@ def test_bit_exact_rerun_and_launch_shapes():
         assert a["stats"]["segments"] == other["stats"]["segments"]
 
 
+def test_bit_exact_with_reserved_ray_blocks():
+    """Above a few million rays the persistent waves take ray ids in reserved
+    blocks; the sums do not depend on which wave or lane traces a ray, so the
+    persistent launch (with reservations), the same launch on another grid,
+    and the plain one-thread-per-ray launch give identical bits."""
+    sc = small_trench()
+    kw = dict(n_rays=20_000_000, sticking=0.5, bc_x=0, bc_y=1, seed=9)
+    print("[gpu] reserved ray blocks, 2e7 rays", flush=True)
+    a = run_scene(sc, **kw)
+    b = run_scene(sc, block_size=64, compact_lanes=40, **kw)
+    c = run_scene(sc, kernel=1, **kw)
+    for other in (b, c):
+        assert np.array_equal(a["raw"], other["raw"])
+        assert np.array_equal(a["counts"], other["counts"])
+        assert a["stats"]["segments"] == other["stats"]["segments"]
+
+
 def test_fixed_point_matches_fp64_atomics():
     sc = small_trench()
     kw = dict(n_rays=100_000, sticking=0.1, bc_x=0, bc_y=1, seed=5)
