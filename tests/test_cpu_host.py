@@ -33,6 +33,37 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTS) == declared
 
 
+@pytest.mark.parametrize("pyname,cname", [("Params", "rti_params"), ("Stats", "rti_stats"),
+                                           ("Instance", "rti_instance")])
+def test_ctypes_structs_match_the_header(tmp_path, pyname, cname):
+    """The Python binding's structs have the C header's size and field
+    offsets (a stale layout makes the library write past the caller's
+    struct)."""
+    cls = getattr(_lib, pyname)
+    fields = [f[0] for f in cls._fields_]
+    # field names in declaration order, from the header's typedef
+    text = open(os.path.join(ROOT, "include", "rti", "rti_c.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    block = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, flags=re.S).group(1)
+    declared = []
+    for decl in block.split(";"):
+        decl = decl.strip()
+        if decl:
+            names = decl.split(None, 1)[1] if not decl.startswith("const") else decl.split(None, 2)[2]
+            declared += [n.strip().lstrip("*").split("[")[0] for n in names.split(",")]
+    assert fields == declared, (fields, declared)
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf(",\\"{f}\\": %zu", offsetof({cname}, {f}));\n' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "rti/rti_c.h"\nint main(void) {\n'
+                   f'  printf("{{\\"size\\": %zu", sizeof({cname}));\n{body}  printf("}}\\n");\n  return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = json.loads(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    assert got["size"] == ctypes.sizeof(cls), (got["size"], ctypes.sizeof(cls))
+    for f in fields:
+        assert got[f] == getattr(cls, f).offset, (f, got[f], getattr(cls, f).offset)
+
+
 def test_default_params_follow_the_reference():
     p = _lib.default_params()
     assert p.n_rays == 1024                      # device.hpp:289
