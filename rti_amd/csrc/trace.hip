@@ -78,7 +78,7 @@ struct trav_state {
   int cur;          // node being visited (postponed leaves: -1 = no node left)
   float curd;       // entry distance of cur (postponed leaves: skipped beyond tbest)
   int sp;           // stack depth
-  uint32_t pend;    // queued leaf primitives (postponed leaves)
+  uint32_t qa;      // LDS byte address of the next free leaf-queue slot (postponed leaves)
   uint32_t pbest;   // closest primitive so far
   bool found;
 };
@@ -355,10 +355,15 @@ constexpr uint32_t kLeafQueue = 8;
 // Ray ids a persistent wave reserves beyond its lanes' needs.
 constexpr unsigned long long kRayReserve = 256;
 
+// The lane's queue is addressed by a running LDS byte address (T.qa) and
+// grows downwards, one slot (block * 4 bytes) per entry, from its empty top
+// at slot kLeafQueue: queueing a leaf costs one subtract beside the write.
 template <int LS>
-__device__ __forceinline__ lds_u32* leaf_slot(uint32_t j, uint32_t stride) {
-  return (lds_u32*)(trace_lds) + (2u * LS * stride + __umul24(j, stride) + threadIdx.x);
+__device__ __forceinline__ uint32_t leaf_queue_top(uint32_t stride) {
+  return (uint32_t)(size_t)((lds_u32*)(trace_lds) + (2u * LS * stride + kLeafQueue * stride + threadIdx.x));
 }
+
+__device__ __forceinline__ lds_u32* lds_word(uint32_t byte_addr) { return (lds_u32*)(size_t)byte_addr; }
 
 // Node phase: visits T.cur unless its entry distance lies beyond the closest
 // hit found since it was chosen, queues its hit leaf children, descends into
@@ -369,15 +374,16 @@ __device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uin
     node4_hits H;
     node4_boxes<COUNT>(P, T, H, C);
     const uint32_t leaves = leaf_mask(H);
-    uint32_t n = T.pend;
+    const uint32_t step = P.block * 4u;
+    uint32_t qa = T.qa;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (leaves & (1u << c)) {
-        *leaf_slot<LS>(n, P.block) = ~sel4(H.ch, (uint32_t)c);
-        ++n;
+        qa -= step;
+        *lds_word(qa) = ~sel4(H.ch, (uint32_t)c);
       }
     }
-    T.pend = n;
+    T.qa = qa;
     if (order_push<LS>(P, T, H, ovf)) return;
   }
   if (!pop4<LS>(P, T, ovf)) T.cur = -1;
@@ -386,8 +392,9 @@ __device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uin
 // Leaf phase: the most recently queued leaf of this lane.
 template <int LS, bool COUNT>
 __device__ __forceinline__ void leaf_visit(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
-  --T.pend;
-  test_prim<COUNT>(P, o, d, *leaf_slot<LS>(T.pend, P.block), T, C);
+  const uint32_t p = *lds_word(T.qa);
+  T.qa += P.block * 4u;
+  test_prim<COUNT>(P, o, d, p, T, C);
 }
 
 // Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
@@ -586,8 +593,10 @@ __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_st
 
 // Start of a segment: the boundary-face distance bounds the closest-hit
 // query, which starts at the instance's root.
+// qtop: the lane's empty leaf-queue address (postponed leaves; unused otherwise).
 template <int W>
-__device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C) {
+__device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C,
+                                              uint32_t qtop) {
   const inst_desc& D = P.desc[R.inst];
   C.seg++;
   R.segs++;
@@ -599,7 +608,7 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.cur = W == 4 ? D.root4 : D.root;
   T.curd = 0.0f;
   T.sp = 0;
-  T.pend = 0;
+  T.qa = qtop;
   const float tiny = 1e-20f;
   // slab factors only cull boxes (every box is padded beyond its
   // primitives), so the hardware reciprocal (1 ulp) suffices
@@ -761,6 +770,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   // does not change any result.
   unsigned long long res_base = 0, res_end = 0;  // reserved ids [res_base, res_end), wave-uniform
   const unsigned long long tail = (unsigned long long)gridDim.x * (P.block >> 6) * 64ull * 16ull;
+  // the lane's leaf queue: empty at qtop, no room for another node's four
+  // leaves below qfull
+  const uint32_t qtop = PL ? leaf_queue_top<LS>(P.block) : 0u;
+  const uint32_t qfull = qtop - (kLeafQueue - 4) * P.block * 4u;
   for (;;) {
     const unsigned long long need = __ballot(!alive && !exhausted);
     if (need) {
@@ -785,7 +798,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
       if (!alive && !exhausted) {
         if (my < P.ray_hi) {
           start_ray(P, my, R);
-          begin_segment<W>(P, R, T, C);
+          begin_segment<W>(P, R, T, C, qtop);
           alive = true;
           tracing = true;
         } else {
@@ -803,17 +816,17 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     if constexpr (PL) {
       for (;;) {
         const unsigned long long tr = __ballot(tracing);
-        const unsigned long long queued = __ballot(tracing && T.pend > 0);
-        const unsigned long long full = __ballot(tracing && T.pend > kLeafQueue - 4);
+        const unsigned long long queued = __ballot(tracing && T.qa != qtop);
+        const unsigned long long full = __ballot(tracing && T.qa < qfull);
         const unsigned long long nodes = __ballot(tracing && T.cur >= 0);
         const bool leaf_phase =
             queued != 0ull && (full != 0ull || nodes == 0ull || __popcll(queued) * 64 >= P.leaf_min * __popcll(tr));
         if (leaf_phase) {
-          if (tracing && T.pend > 0) leaf_visit<LS, COUNT>(P, R.o, R.d, T, C);
+          if (tracing && T.qa != qtop) leaf_visit<LS, COUNT>(P, R.o, R.d, T, C);
         } else {
           if (tracing && T.cur >= 0) S.template visit<COUNT>(P, T, C);
         }
-        tracing = tracing && (T.cur >= 0 || T.pend > 0);
+        tracing = tracing && (T.cur >= 0 || T.qa != qtop);
         if (__popcll(__ballot(tracing)) <= still) break;
       }
     } else {
@@ -825,7 +838,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     if (alive && !tracing) {
       alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
       if (alive) {
-        begin_segment<W>(P, R, T, C);
+        begin_segment<W>(P, R, T, C, qtop);
         tracing = true;
       }
     }
@@ -844,7 +857,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   if (gid < P.ray_hi) {
     start_ray(P, gid, R);
     for (;;) {
-      begin_segment<W>(P, R, T, C);
+      begin_segment<W>(P, R, T, C, 0u);
       while (!S.template step<COUNT>(P, R.o, R.d, T, C)) {
       }
       if (!finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C)) break;
