@@ -287,6 +287,15 @@ __global__ void k_set_u32(uint32_t* p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Record p's last float4 carries the start of its neighbour row in w (for
+// p = n, the sentinel record after the store: the end of the last row), so
+// that the shading of a surface hit finds the row bounds beside the normal it
+// loads instead of in a separate array.
+__global__ void k_link_rows(float4* prims, uint32_t shift, const uint32_t* nb_ptr, uint32_t n) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p <= n) prims[(((size_t)p + 1) << shift) - 1].w = __uint_as_float(nb_ptr[p]);
+}
+
 __global__ void k_set_one_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
 // Binary tree topology. Default: PLOC for triangle meshes (measured +1.5 %
@@ -324,7 +333,8 @@ int prepare(rti_ctx* ctx) {
   bool discs_only = true;
   for (auto& I : ctx->inst) discs_only = discs_only && I.kind == 0;
   ctx->prim_shift = discs_only ? rti_dev::kPrimShiftDiscs : rti_dev::kPrimShiftMixed;
-  HIP_TRY(ctx, ctx->prims.ensure((np * 16) << ctx->prim_shift));
+  // one sentinel record after the last primitive (its neighbour-row end)
+  HIP_TRY(ctx, ctx->prims.ensure(((np + 1) * 16) << ctx->prim_shift));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
   HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
@@ -398,6 +408,9 @@ int prepare(rti_ctx* ctx) {
     HIP_TRY(ctx, hipGetLastError());
   }
   ctx->edges_total = edges;
+  k_link_rows<<<(unsigned)((np + 1 + 255) / 256), 256, 0, ctx->stream>>>(ctx->prims.as<float4>(), ctx->prim_shift,
+                                                                         ctx->nb_ptr.as<uint32_t>(), (uint32_t)np);
+  HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, ctx->nb_col.ensure(std::max<uint64_t>(edges, 1) * 4));
   st.nb_col = ctx->nb_col.as<uint32_t>();
   for (size_t k = 0; k < ctx->inst.size(); ++k) {
@@ -536,7 +549,6 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.prim_shift = ctx->prim_shift;
   L.nodes = ctx->nodes.as<bvh_node>();
   L.nodes4 = ctx->nodes4.as<bvh4_node>();
-  L.nb_ptr = ctx->nb_ptr.as<uint32_t>();
   L.nb_col = ctx->nb_col.as<uint32_t>();
   L.acc = ctx->acc.as<unsigned long long>();
   L.adj = adj ? ctx->adj.as<unsigned long long>() : nullptr;

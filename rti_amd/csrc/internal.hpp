@@ -88,8 +88,7 @@ struct trace_launch {
   uint32_t prim_shift;              // log2 float4 per primitive record
   const rti_dev::bvh_node* nodes;
   const rti_dev::bvh4_node* nodes4;
-  const uint32_t* nb_ptr;
-  const uint32_t* nb_col;
+  const uint32_t* nb_col;           // neighbour columns; row bounds are in the prims records
   unsigned long long* acc;          // kAccStride per primitive: sum (fixed point or double bits), sum of squares, count
   unsigned long long* adj;          // 2 per primitive: d/ds, d/dn (signed fixed point); may be null
   double* mom;                      // 2 per primitive: sums of deposit^3, ^4 (FP64 atomics); may be null

@@ -12,10 +12,15 @@
 //                    k = 2  discs: unused;               triangles: e2 = v2 - v0
 //                    k = 3  unit normal (discs as given; triangles from the winding)
 //                  The last float4 of a record, k = (1 << s) - 1, is always
-//                  the unit normal.
+//                  the unit normal, with the start of the primitive's
+//                  neighbour row (nb_ptr[p], u32 bits) in w. One sentinel
+//                  record after the last primitive holds only that w (the
+//                  end of the last row), so row p is [w of p, w of p + 1).
 //   nodes4[i]      bvh4_node 64 B, four quantised child boxes (the traversal tree)
 //   nodes[i]       bvh_node 64 B, both children's boxes in the parent (the build's tree)
-//   nb_ptr/nb_col  neighbour CSR over absolute prim ids (discs)
+//   nb_ptr/nb_col  neighbour CSR over absolute prim ids (discs; the trace
+//                  reads the row bounds from the records, nb_ptr serves the
+//                  build, the smoothing and the exports)
 //   acc[4p + k]    u64  k = 0 sum of deposits, 1 sum of squares (fixed point,
 //                  or FP64 bits), 2 deposit count, 3 unused: one 32-byte
 //                  record, so a deposit's three atomics share a cache line

@@ -49,7 +49,6 @@ struct tparams {
   uint32_t prim_shift;
   const bvh_node* __restrict__ nodes;
   const bvh4_node* __restrict__ nodes4;
-  const uint32_t* __restrict__ nb_ptr;
   const uint32_t* __restrict__ nb_col;
   unsigned long long* acc;  // kAccStride per primitive: sum, sum of squares, count
   unsigned long long* adj;  // 2 per primitive: d/ds, d/dn (adjoint only)
@@ -664,9 +663,12 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
   const float scale = D.fx_scale;
   deposit<ADJ, FP64ACC, COUNT>(P, prim, drop, scale, dds, ddn, C);
   if (disc) {  // check_for_additional_intersections (tracer.hpp:339-383)
-    // the column indices of up to four neighbours are loaded together, so
-    // the records wait for one index latency per four neighbours
-    const uint32_t q0 = P.nb_ptr[prim], q1 = P.nb_ptr[prim + 1];
+    // the neighbour row starts at the w word of this record's last float4
+    // (loaded with the normal) and ends at that of the next record; the
+    // column indices of up to four neighbours are loaded together, so the
+    // records wait for one index latency per four neighbours
+    const uint32_t q0 = __float_as_uint(N4.w);
+    const uint32_t q1 = __float_as_uint(P.prims[((size_t)(prim + 2) << P.prim_shift) - 1].w);
     for (uint32_t q = q0; q < q1; q += 4) {
       uint32_t j[4];
 #pragma unroll
@@ -933,7 +935,6 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.prim_shift = L.prim_shift;
   P.nodes = L.nodes;
   P.nodes4 = L.nodes4;
-  P.nb_ptr = L.nb_ptr;
   P.nb_col = L.nb_col;
   P.acc = L.acc;
   P.adj = L.adj;
