@@ -147,6 +147,48 @@ def cpu_baseline(cases, budget_s=20.0, adjoint=False):
                       f"{len(cases)} cases of the same geometry: {segs} segments in {secs:.1f} s"}
 
 
+def radiosity_report(ctx, cases):
+    """Hole workload: the mean incident flux on 20 wall bands (depth 0.25
+    each) of every (s, n) case of the last run, against the FP64 radiosity
+    solution of the same hole committed in tests/golden/radiosity_hole_bands.json.
+    Incident flux of a triangle = raw / (s * area) * source area / rays; band
+    sigma from the per-triangle sigmas; bound per band: 3 sigma + 1 % (the
+    320-gon mesh against the true cylinder)."""
+    with open(os.path.join(ROOT, "tests", "golden", "radiosity_hole_bands.json")) as fh:
+        ref = json.load(fh)
+    sc = cases[0]["scene"]
+    v = sc["verts"][sc["tris"]].astype(np.float64)
+    c = v.mean(1)
+    area = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    top = float(sc["verts"][:, 2].max())
+    depth = top - c[:, 2]
+    wall = (depth > 1e-6) & (depth < sc["depth"] - 1e-6) & (np.abs(np.hypot(c[:, 0], c[:, 1]) - sc["radius"]) < 0.01)
+    lo, hi = sc["verts"].min(0), sc["verts"].max(0)
+    a_src = float((hi[0] - lo[0]) * (hi[1] - lo[1]))
+    edges = ref["band_edges_depth"]
+    out, ok = [], True
+    for k, case in enumerate(cases):
+        s, n_exp, n = case["sticking"], case.get("cos_exponent", 1.0), case["n_rays"]
+        want = next(r["wall_band_flux"] for r in ref["cases"] if r["sticking"] == s and r["cos_exponent"] == n_exp)
+        f = ctx.batch_flux(k)
+        g = f["raw"] / (s * area) * a_src / n
+        rel, zs, within = [], [], True
+        for b in range(len(want)):
+            m = wall & (depth >= edges[b]) & (depth < edges[b + 1])
+            w = area[m]
+            est = float(np.sum(g[m] * w) / w.sum())
+            sig = float(np.sqrt(np.sum((f["sigma"][m] / s * a_src / n) ** 2)) / w.sum())
+            err = abs(est - want[b])
+            rel.append(err / want[b])
+            zs.append(err / sig if sig > 0 else float("inf"))
+            within = within and err < 3 * sig + 0.01 * want[b]
+        ok = ok and within
+        out.append({"s": s, "n": n_exp, "max_rel_err": round(max(rel), 5), "mean_rel_err": round(float(np.mean(rel)), 5),
+                    "max_err_over_sigma": round(max(zs), 2), "within_3sigma_plus_1pct": within})
+    return {"reference": "FP64 radiosity of the same hole (tests/golden/radiosity_hole_bands.json)",
+            "bands": "20 wall bands, depth 0.25", "cases": out, "all_within": ok}
+
+
 def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
     """Summed d(raw)/ds and d(raw)/dn of the adjoint pass against central
     differences of the summed raw flux with common random numbers, repeated
@@ -247,6 +289,11 @@ def main():
     value = segments / (ms_per_step / 1e3)
 
     extra = {}
+    if args.workload == "hole":
+        # flux of the last timed step against the FP64 radiosity solution
+        ctx.batch_run()
+        extra["flux_vs_radiosity"] = radiosity_report(ctx, cases)
+        log(f"flux vs radiosity: {extra['flux_vs_radiosity']}")
     if args.workload == "rough":
         # same paths, FP64 atomic accumulation: the fixed point's error
         ctx.batch_run()

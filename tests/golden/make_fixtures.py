@@ -15,6 +15,12 @@ JSON files).
   aspect-45 cylinder, radius 0.5, depth 45, s in {1, 0.1, 0.01}
   (verification/radiosity/analytical_solution/original/*.txt; rti is
   BSD-3-licensed, attribution kept in the file).
+* radiosity_hole_bands.json: the FP64 radiosity solution (oracle/radiosity.py,
+  itself pinned by radiosity_ar45.json) of the benchmark's hole, radius 0.5,
+  depth 5, for s in {0.01, 0.1, 1} x n in {1, 10}: the incident flux on 20
+  wall bands of depth 0.25 (means of 20 radiosity elements each), which the
+  benchmark compares its full-size fluxes with. Needs no reference tree
+  (``python tests/golden/make_fixtures.py radiosity_hole_bands.json``).
 
 The inputs are transcribed from the reference test sources; the expected
 values are those the reference asserts.
@@ -140,9 +146,30 @@ def radiosity():
     return out
 
 
+def hole_bands():
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from oracle import radiosity as rad
+    out = {"source": "oracle/radiosity.py solve_hole (FP64 radiosity, ring-to-ring view factors, cos^n source)",
+           "geometry": "cylindrical hole radius 0.5, depth 5, opening at z = 5",
+           "quantity": "mean incident flux per wall band relative to the source flux density",
+           "band_edges_depth": None, "cases": []}
+    for s in (0.01, 0.1, 1.0):
+        for n in (1.0, 10.0):
+            ref = rad.solve_hole(0.5, 5.0, s, cos_exponent=n, n_wall=400, n_floor=10)
+            out["band_edges_depth"] = ref["z"][::20].tolist()
+            out["cases"].append({"sticking": s, "cos_exponent": n,
+                                 "wall_band_flux": ref["wall"].reshape(20, 20).mean(1).tolist()})
+    return out
+
+
 if __name__ == "__main__":
-    for name, fn in [("local_intersector_kats.json", local_intersector), ("area_inside_kats.json", area_inside),
-                     ("radiosity_ar45.json", radiosity)]:
+    import sys
+    jobs = [("local_intersector_kats.json", local_intersector), ("area_inside_kats.json", area_inside),
+            ("radiosity_ar45.json", radiosity), ("radiosity_hole_bands.json", hole_bands)]
+    if len(sys.argv) > 1:
+        jobs = [j for j in jobs if j[0] in sys.argv[1:]]
+    for name, fn in jobs:
         with open(os.path.join(HERE, name), "w") as fh:
             json.dump(fn(), fh, indent=1)
         print("wrote", name)

@@ -178,6 +178,20 @@ def test_radiosity_reproduces_reference_ar45_solution(s):
     assert rel.max() < 1e-6, rel.max()
 
 
+def test_hole_band_fixture_matches_the_solver():
+    """The radiosity band fluxes the benchmark compares with
+    (tests/golden/radiosity_hole_bands.json) are the solver's for the
+    benchmark's hole."""
+    with open(os.path.join(ROOT, "tests", "golden", "radiosity_hole_bands.json")) as fh:
+        fx = json.load(fh)
+    assert [(c["sticking"], c["cos_exponent"]) for c in fx["cases"]] == [
+        (s, n) for s in (0.01, 0.1, 1.0) for n in (1.0, 10.0)]
+    for case in fx["cases"][2:4]:
+        ref = radiosity.solve_hole(0.5, 5.0, case["sticking"], cos_exponent=case["cos_exponent"], n_wall=400, n_floor=10)
+        np.testing.assert_allclose(ref["wall"].reshape(20, 20).mean(1), case["wall_band_flux"], rtol=1e-12)
+        np.testing.assert_allclose(ref["z"][::20], fx["band_edges_depth"], rtol=0, atol=1e-12)
+
+
 def test_radiosity_energy_balance():
     """Absorbed + escaped = entered, for the hole and the trench."""
     for s in (1.0, 0.3, 0.02):
