@@ -64,6 +64,34 @@ def test_trench_common_random_numbers(s):
     assert rel_seg < 1e-3, (g["stats"]["segments"], o["segments"])
 
 
+@pytest.mark.parametrize("which,s", [("trench", 1.0), ("trench", 0.1), ("hole_mesh", 1.0), ("hole_mesh", 0.1)])
+def test_fp32_trace_vs_fp64_oracle_common_random_numbers(which, s):
+    """The error of FP32 traversal and intersection plus 64-bit fixed-point
+    accumulation against the FP64 oracle (double-precision geometry tests,
+    FP64 sums) on the same Philox streams: per-element sums far inside the
+    Monte Carlo error, most hit counts equal, totals within 1e-4 relative
+    (FP32 rounding forks a path now and then; the oracle's own FP32 build
+    differs from its FP64 build by about as much)."""
+    sc = small_trench() if which == "trench" else small_hole_mesh()
+    kw = dict(bc_x=0, bc_y=1) if which == "trench" else dict(bc_x=1, bc_y=1)
+    n = 200_000
+    g = _gpu(sc, n, s, **kw)
+    o = oracle_run(sc, n, s, fp64=True, tie_rank=tie_rank(g["perm"]), **kw)
+    sig = np.maximum(np.sqrt(np.maximum(o["s2"] - o["s1"] ** 2 / n, 0)), 1e-12)
+    d = np.abs(g["raw"] - o["s1"]) / sig
+    m = o["s1"] > 0
+    rel = np.abs(g["raw"][m] - o["s1"][m]) / o["s1"][m]
+    same = np.mean(g["counts"] == o["cnt"])
+    total = abs(g["raw"].sum() - o["s1"].sum()) / o["s1"].sum()
+    print(f"[gpu] {which} s={s}: |diff|/sigma q99 {np.quantile(d, 0.99):.3g} max {d.max():.3g}; "
+          f"relative error per element mean {rel.mean():.3g} max {rel.max():.3g}; total {total:.3g}; "
+          f"counts equal {same:.4f}", flush=True)
+    assert np.quantile(d, 0.99) < 0.5
+    assert same > (0.99 if s == 1.0 else 0.9)
+    assert total < 1e-4
+    assert abs(g["stats"]["segments"] - o["segments"]) < 1e-3 * o["segments"]
+
+
 @pytest.mark.parametrize("s,bcx,bcy", [(0.1, 0, 1), (0.02, 1, 1), (0.5, 0, 0)])
 def test_trench_independent_seeds_within_3sigma(s, bcx, bcy):
     sc = small_trench()
