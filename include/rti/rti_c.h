@@ -123,6 +123,9 @@ typedef struct rti_params {
                              -1 = test leaves at each node visit                       */
   int32_t tree_builder;   /* rti_tree_builder; 0 = default (PLOC for triangle meshes,
                              LBVH for disc clouds)                                      */
+  int32_t refill_lanes;   /* persistent kernel: a wave replaces ended rays once this many
+                             of its lanes wait for one, or when none is alive (1..64);
+                             0 = default (1, at every shading pass)                     */
 } rti_params;
 
 typedef struct rti_stats {

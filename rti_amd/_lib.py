@@ -57,7 +57,7 @@ class Params(ctypes.Structure):
         ("adjoint", ctypes.c_int32), ("accum", ctypes.c_int32), ("block_size", ctypes.c_int32),
         ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
         ("bvh_width", ctypes.c_int32), ("moments", ctypes.c_int32), ("source", ctypes.c_int32),
-        ("leaf_batch", ctypes.c_int32), ("tree_builder", ctypes.c_int32),
+        ("leaf_batch", ctypes.c_int32), ("tree_builder", ctypes.c_int32), ("refill_lanes", ctypes.c_int32),
     ]
 
 

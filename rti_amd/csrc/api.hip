@@ -154,6 +154,7 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
   if (p.leaf_batch < -1 || p.leaf_batch > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..64");
+  if (p.refill_lanes < 0 || p.refill_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "refill_lanes must be 0..64");
   if (p.tree_builder != 0 && p.tree_builder != RTI_TREE_LBVH && p.tree_builder != RTI_TREE_PLOC)
     return fail(ctx, RTI_ERR_INVALID_ARG, "tree_builder must be 0, RTI_TREE_LBVH or RTI_TREE_PLOC");
   if (p.bvh_width != 0 && p.bvh_width != 2 && p.bvh_width != 4)
@@ -565,6 +566,7 @@ int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool
   L.compact = ctx->params.compact_lanes;
   L.width = cfg.width;
   L.leaf_batch = cfg.leaf;
+  L.refill_lanes = ctx->params.refill_lanes ? ctx->params.refill_lanes : 1;
   if (kernel == RTI_KERNEL_PERSISTENT) {
     L.ray_lo = 0;
     L.ray_hi = ctx->rays_total;

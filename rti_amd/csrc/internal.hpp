@@ -106,6 +106,7 @@ struct trace_launch {
   int width;                        // traversal tree: 2 binary, 4 four-wide
   int leaf_batch;                   // persistent four-wide: postponed leaf tests, a leaf phase once this many
                                     // 64ths of the tracing lanes hold a queued leaf (1..64); 0 = at each node
+  int refill_lanes;                 // persistent: lanes waiting for a new ray before a wave refills (1..64)
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
 hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,

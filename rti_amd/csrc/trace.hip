@@ -57,6 +57,7 @@ struct tparams {
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
   int leaf_min;  // postponed leaves: 64ths of the tracing lanes that must hold one
+  int refill_min;  // lanes waiting for a new ray before a wave refills (1..64)
   uint32_t block;  // threads per block (the LDS slot stride), uniform
 };
 
@@ -777,8 +778,11 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   const uint32_t qtop = PL ? leaf_queue_top<LS>(P.block) : 0u;
   const uint32_t qfull = qtop - (kLeafQueue - 4) * P.block * 4u;
   for (;;) {
+    // ended rays are replaced once refill_min lanes wait for one (a refill
+    // costs the wave the same whatever the number of lanes it serves), or
+    // when no lane is alive
     const unsigned long long need = __ballot(!alive && !exhausted);
-    if (need) {
+    if (need && (__popcll(need) >= P.refill_min || __ballot(alive) == 0ull)) {
       const unsigned long long cnt = (unsigned long long)__popcll(need);
       const unsigned long long rank = (unsigned long long)__popcll(need & lt);
       const unsigned long long have = res_end - res_base;
@@ -943,6 +947,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.stats = L.stats;
   P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();
   P.leaf_min = L.leaf_batch > 64 ? 64 : L.leaf_batch;
+  P.refill_min = L.refill_lanes < 1 ? 1 : (L.refill_lanes > 64 ? 64 : L.refill_lanes);
   const bool persistent = L.kernel == 0;
   const bool postpone = persistent && L.width == 4 && L.leaf_batch > 0;
   const bool count = L.count != 0;

@@ -7,7 +7,8 @@ closest-hit query before the wave shades and refills), then the resident
 waves per CU (0 = as many as registers and LDS allow; fewer can suit
 workloads whose working set outgrows the caches), then the LDS stack depth,
 then the block size, then ``leaf_batch`` (-1: leaf tests at each node visit;
-64: postponed leaf tests in wave-wide leaf phases). Results are cached in a JSON file keyed by the
+64: postponed leaf tests in wave-wide leaf phases), then ``refill_lanes``
+(lanes of a wave that must wait for a new ray before the wave refills). Results are cached in a JSON file keyed by the
 batch signature, so a repeated workload is not re-tuned.
 """
 import hashlib
@@ -20,6 +21,7 @@ COMPACT = (40, 48, 56, 60, 64)
 LDS = (4, 8, 16)
 BLOCK = (128, 256)
 LEAF = (-1, 64)
+REFILL = (1, 8, 16)
 WAVES = (0, 20, 24)
 
 
@@ -36,7 +38,7 @@ def signature(items):
 def _rate(ctx, knobs):
     ctx.set_params(**knobs)
     ctx.batch_replay(1)          # capture + warm
-    ctx.batch_replay(1)
+    ctx.batch_replay(2)          # mean of two replays (run-to-run spread ~2 %)
     st = ctx.batch_stats()
     return float(ctx.counters()[0]) / (st["trace_ms"] / 1e3)
 
@@ -51,7 +53,7 @@ def default_fraction(items, sample_rays=5e7, full_up_to=1e8):
 
 
 def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
-    """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size, leaf_batch} and their rate."""
+    """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size, leaf_batch, refill_lanes} and their rate."""
     key = signature(items)
     if cache and os.path.exists(cache):
         with open(cache) as fh:
@@ -61,13 +63,14 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     if fraction is None:
         fraction = default_fraction(items)
     small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
-    best = {"compact_lanes": 56, "waves_per_cu": 0, "lds_stack": 8, "block_size": 256, "leaf_batch": 0}
+    best = {"compact_lanes": 56, "waves_per_cu": 0, "lds_stack": 8, "block_size": 256, "leaf_batch": 0,
+            "refill_lanes": 1}
     trials = []
     with Context() as ctx:
         ctx.batch_set(small)
         ctx.prepare()
         for name, values in (("compact_lanes", COMPACT), ("waves_per_cu", WAVES), ("lds_stack", LDS),
-                             ("block_size", BLOCK), ("leaf_batch", LEAF)):
+                             ("block_size", BLOCK), ("leaf_batch", LEAF), ("refill_lanes", REFILL)):
             scores = {}
             for v in values:
                 knobs = dict(best, **{name: v})
@@ -90,5 +93,5 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
 
 
 def knobs(tuned):
-    return {k: tuned[k] for k in ("compact_lanes", "waves_per_cu", "lds_stack", "block_size", "leaf_batch")
-            if k in tuned}
+    return {k: tuned[k] for k in ("compact_lanes", "waves_per_cu", "lds_stack", "block_size", "leaf_batch",
+                                  "refill_lanes") if k in tuned}

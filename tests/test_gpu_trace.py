@@ -141,7 +141,8 @@ def test_bit_exact_with_reserved_ray_blocks():
     a = run_scene(sc, **kw)
     b = run_scene(sc, block_size=64, compact_lanes=40, **kw)
     c = run_scene(sc, kernel=1, **kw)
-    for other in (b, c):
+    d = run_scene(sc, refill_lanes=16, **kw)  # rays replaced in groups of 16 lanes
+    for other in (b, c, d):
         assert np.array_equal(a["raw"], other["raw"])
         assert np.array_equal(a["counts"], other["counts"])
         assert a["stats"]["segments"] == other["stats"]["segments"]

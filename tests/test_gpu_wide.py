@@ -168,6 +168,9 @@ def test_invalid_leaf_batch_is_rejected():
         for v in (-2, 65):
             with pytest.raises(Exception):
                 ctx.set_params(leaf_batch=v)
+        for v in (-1, 65):
+            with pytest.raises(Exception):
+                ctx.set_params(refill_lanes=v)
 
 
 def test_invalid_tree_builder_is_rejected():
