@@ -103,6 +103,33 @@ def test_trench_independent_seeds_within_3sigma(s, bcx, bcy):
     check_z(z)
 
 
+def test_config1_trench_per_element_within_3sigma():
+    """The reference's CPU-runnable case at its own size: the disc-cloud
+    trench of width 1, depth 4 on the 0.02 grid (51,051 discs), reflective x,
+    periodic y, source at z = 4.1, s = 0.1, 1e6 rays. GPU and oracle with
+    independent seeds agree per element within 3 sigma."""
+    sc = geometry.trench_discs()
+    n = 1_000_000
+    g = _gpu(sc, n, 0.1, seed=21, bc_x=0, bc_y=1, src_z=4.1)
+    o = oracle_run(sc, n, 0.1, seed=4321, bc_x=0, bc_y=1, src_z=4.1)
+    so = np.sqrt(np.maximum(o["s2"] - o["s1"] ** 2 / n, 0))
+    z = zscores(g["raw"], g["sigma"], o["s1"], so, min_count=30, ca=g["counts"], cb=o["cnt"])
+    print(f"[gpu] c1 trench: {z.size} elements, z stats {check_z(z)}", flush=True)
+
+
+@pytest.mark.parametrize("s,nexp,n", [(0.1, 1.0, 8_000_000), (1.0, 10.0, 8_000_000), (0.01, 1.0, 2_000_000)])
+def test_config2_hole_mesh_per_element_within_3sigma(s, nexp, n):
+    """The full triangle-mesh hole of the benchmark (199,360 triangles,
+    radius 0.5, depth 5, 2 x 2 periodic cell): GPU and oracle with
+    independent seeds agree per triangle within 3 sigma."""
+    sc = geometry.hole_triangles()
+    g = _gpu(sc, n, s, seed=31, cos_exponent=nexp, bc_x=1, bc_y=1)
+    o = oracle_run(sc, n, s, seed=7654, cos_exponent=nexp, bc_x=1, bc_y=1)
+    so = np.sqrt(np.maximum(o["s2"] - o["s1"] ** 2 / n, 0))
+    z = zscores(g["raw"], g["sigma"], o["s1"], so, min_count=30, ca=g["counts"], cb=o["cnt"])
+    print(f"[gpu] c2 hole s={s} n={nexp}: {z.size} elements, z stats {check_z(z)}", flush=True)
+
+
 def test_triangle_mesh_independent_seeds_within_3sigma():
     sc = small_hole_mesh()
     n = 400_000
