@@ -20,7 +20,8 @@
 //     lies beyond the closest hit found meanwhile is dropped without a fetch;
 //   binary: both child boxes in the parent (one 64-byte node per visit).
 // The per-lane stack keeps its top LS entries in LDS and the rest in private
-// memory.
+// memory. In the persistent kernel on the four-wide tree, hit leaves can be
+// queued per lane in LDS and tested in wave-wide leaf phases (leaf_batch).
 // Accumulation: per-primitive 64-bit integer atomics of round(v * 2^shift),
 // so totals are independent of the order of deposits (bit-exact run to run).
 
