@@ -585,7 +585,14 @@ __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, cons
     e[k] = c4_exponent(lo[k], hi[k]);
     sc[k] = __uint_as_float(e[k] << 23);
   }
+  // Slots beyond the m children hold an inverted box (low planes 255, high
+  // planes 0), which no slab test enters, and a primitive of this subtree as
+  // a leaf, so that even a test that rounded into one would only test a
+  // primitive of the ray's own geometry again (never push a stack entry the
+  // stack bound does not count).
   uint32_t ql[3] = {0, 0, 0}, qh[3] = {0, 0, 0};
+  for (int c = m; c < 4; ++c)
+    for (int k = 0; k < 3; ++k) ql[k] |= 0xffu << (8 * c);
   int child[4] = {0, 0, 0, 0};
   uint32_t j = 0;
   const uint32_t first = base + width + offsets[i];
@@ -615,6 +622,9 @@ __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, cons
   out.hi_z = qh[2];
   out.pad0 = 0;
   out.pad1 = 0;
+  int spare = ref[0];
+  while (spare >= 0) spare = nodes[spare].child.x;
+  for (int c = m; c < 4; ++c) child[c] = spare;
   out.child = make_int4(child[0], child[1], child[2], child[3]);
   nodes4[node4_begin + base + i] = out;
 }

@@ -208,8 +208,9 @@ __device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
   return hit ? ((__float_as_uint(tmin) & ~3u) | c) : 0xffffffffu;
 }
 
-// Child boxes of the four-wide node T.cur: entry distance and hit (child
-// present, slab interval non-empty within [tnear, tbest]) per child.
+// Child boxes of the four-wide node T.cur: entry distance and hit (slab
+// interval non-empty within [tnear, tbest]) per child. Empty slots hold an
+// inverted box, which no ray enters.
 //
 // Slab distances: the plane of byte q is org + q * 2^e, and
 //   t = (org + q * 2^e) * inv - o * inv = q * (2^e * inv) + (org * inv - o * inv),
@@ -234,7 +235,6 @@ __device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& 
   const float bx = fmaf(__uint_as_float(q0.x), inv.x, -oi.x);
   const float by = fmaf(__uint_as_float(q0.y), inv.y, -oi.y);
   const float bz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
-  const uint32_t m = meta >> 24;
   // entry / exit planes per axis follow the sign of the direction
   const uint32_t nx = inv.x >= 0.0f ? q1.x : q1.w, fx = inv.x >= 0.0f ? q1.w : q1.x;
   const uint32_t ny = inv.y >= 0.0f ? q1.y : q2.x, fy = inv.y >= 0.0f ? q2.x : q1.y;
@@ -250,7 +250,7 @@ __device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& 
     const float tfz = fmaf((float)((fz >> sh) & 0xffu), az, bz);
     H.tmin[c] = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, T.tnear));
     const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, T.tbest));
-    H.hit[c] = (uint32_t)c < m && H.tmin[c] <= tmax;
+    H.hit[c] = H.tmin[c] <= tmax;
   }
 }
 
@@ -268,13 +268,15 @@ __device__ __forceinline__ uint32_t leaf_mask(const node4_hits& H) {
 // each with its entry distance, so that a popped entry beyond the closest hit
 // found meanwhile is dropped without fetching it. Children entered beyond
 // tbest are not kept. Returns false when no internal child is kept.
-template <int LS>
+// RECHECK: tbest may have dropped since the boxes were tested (leaves
+// tested in between), so entry distances are compared with it again.
+template <int LS, bool RECHECK>
 __device__ __forceinline__ bool order_push(const tparams& P, trav_state& T, const node4_hits& H, uint64_t* ovf) {
   const int4 ch = H.ch;
-  uint32_t k0 = sort_key(H.hit[0] && ch.x >= 0 && H.tmin[0] <= T.tbest, H.tmin[0], 0);
-  uint32_t k1 = sort_key(H.hit[1] && ch.y >= 0 && H.tmin[1] <= T.tbest, H.tmin[1], 1);
-  uint32_t k2 = sort_key(H.hit[2] && ch.z >= 0 && H.tmin[2] <= T.tbest, H.tmin[2], 2);
-  uint32_t k3 = sort_key(H.hit[3] && ch.w >= 0 && H.tmin[3] <= T.tbest, H.tmin[3], 3);
+  uint32_t k0 = sort_key(H.hit[0] && ch.x >= 0 && (!RECHECK || H.tmin[0] <= T.tbest), H.tmin[0], 0);
+  uint32_t k1 = sort_key(H.hit[1] && ch.y >= 0 && (!RECHECK || H.tmin[1] <= T.tbest), H.tmin[1], 1);
+  uint32_t k2 = sort_key(H.hit[2] && ch.z >= 0 && (!RECHECK || H.tmin[2] <= T.tbest), H.tmin[2], 2);
+  uint32_t k3 = sort_key(H.hit[3] && ch.w >= 0 && (!RECHECK || H.tmin[3] <= T.tbest), H.tmin[3], 3);
   {
     uint32_t a = min(k0, k1), b = max(k0, k1);
     k0 = a; k1 = b;
@@ -342,7 +344,7 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
     leaves &= leaves - 1;
     test_prim<COUNT>(P, o, d, ~sel4(H.ch, c), T, C);
   }
-  if (order_push<LS>(P, T, H, ovf)) return false;
+  if (order_push<LS, true>(P, T, H, ovf)) return false;
   return !pop4<LS>(P, T, ovf);
 }
 
@@ -385,7 +387,7 @@ __device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uin
       }
     }
     T.qa = qa;
-    if (order_push<LS>(P, T, H, ovf)) return;
+    if (order_push<LS, false>(P, T, H, ovf)) return;
   }
   if (!pop4<LS>(P, T, ovf)) T.cur = -1;
 }
