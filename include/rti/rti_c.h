@@ -214,9 +214,11 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
                       float* prim_a);
 
 /* The four-wide traversal tree of the (single) geometry: 64-byte records
- * { float org[3]; uint32 meta = e_x | e_y << 8 | e_z << 16 | nchild << 24;
- *   uint32 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z, pad, pad; int32 child[4] },
- * child c's box planes fmaf(byte c of the plane word, 2^(e - 127), org);
+ * { float org[3], s_x; uint32 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z;
+ *   float s_y, s_z; int32 child[4] },
+ * child c's box planes fmaf(byte c of the plane word, |s|, org) with s a
+ * power of two; nchild - 1 = (s_y < 0) | (s_z < 0) << 1; slots beyond
+ * nchild hold an inverted box and a leaf of the node's subtree;
  * child >= 0 is a node index relative to this geometry, < 0 is ~stored
  * primitive position. Root is record 0. The reference's Embree BVH is
  * internal to Embree and has no such export; this one serves validation. */

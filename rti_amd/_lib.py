@@ -78,9 +78,17 @@ class Stats(ctypes.Structure):
 
 
 # Four-wide traversal-tree record (64 bytes): child c's box planes are
-# fmaf(byte c of the plane word, 2^(e - 127), org) with e the axis's byte of meta.
-WIDE_NODE = np.dtype([("org", "<f4", 3), ("meta", "<u4"), ("lo", "<u4", 3), ("hi", "<u4", 3), ("pad", "<u4", 2),
+# fmaf(byte c of the plane word, |s|, org) with s the axis's power-of-two
+# scale; the signs of s_y and s_z hold nchild - 1 (bits 0 and 1).
+WIDE_NODE = np.dtype([("org", "<f4", 3), ("sx", "<f4"), ("lo", "<u4", 3), ("hi", "<u4", 3), ("syz", "<f4", 2),
                       ("child", "<i4", 4)])
+
+
+def wide_scales(node):
+    """(|s_x|, |s_y|, |s_z|) and the child count of a four-wide record."""
+    s = np.abs(np.array([node["sx"], node["syz"][0], node["syz"][1]], dtype=np.float32))
+    m = 1 + int(node["syz"][0] < 0) + 2 * int(node["syz"][1] < 0)
+    return s, m
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)

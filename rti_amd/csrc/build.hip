@@ -613,15 +613,15 @@ __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, cons
   out.ox = lo[0];
   out.oy = lo[1];
   out.oz = lo[2];
-  out.meta = e[0] | (e[1] << 8) | (e[2] << 16) | ((uint32_t)m << 24);
+  out.sx = sc[0];
+  out.sy = ((m - 1) & 1) ? -sc[1] : sc[1];
+  out.sz = ((m - 1) & 2) ? -sc[2] : sc[2];
   out.lo_x = ql[0];
   out.lo_y = ql[1];
   out.lo_z = ql[2];
   out.hi_x = qh[0];
   out.hi_y = qh[1];
   out.hi_z = qh[2];
-  out.pad0 = 0;
-  out.pad1 = 0;
   int spare = ref[0];
   while (spare >= 0) spare = nodes[spare].child.x;
   for (int c = m; c < 4; ++c) child[c] = spare;
@@ -638,7 +638,7 @@ __global__ void k_c4_stack(const bvh4_node* nodes4, uint32_t node4_begin, uint32
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= width) return;
   const bvh4_node nd = nodes4[node4_begin + base + i];
-  const int m = (int)(nd.meta >> 24);
+  const int m = bvh4_nchild(nd);
   const int ch[4] = {nd.child.x, nd.child.y, nd.child.z, nd.child.w};
   uint32_t k = 0, deepest = 0;
   for (int c = 0; c < m; ++c) {

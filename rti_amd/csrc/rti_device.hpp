@@ -54,19 +54,27 @@ static_assert(sizeof(bvh_node) == 64, "node must be one 64-byte record");
 
 // Four-wide node, collapsed from the binary tree, with its children's boxes
 // quantised to 8 bits per plane in the frame of the node's own box:
-//   plane = fmaf(q, 2^(e_axis - 127), org_axis),
-// where q is byte c (child c) of the plane's word and e_axis the biased
-// exponent in `meta`. The build rounds lower planes down and upper planes up
-// through exactly this expression, so every quantised box contains the
-// child's exact box. Children 0..nchild-1 are valid; child[c] >= 0 is a
+//   plane = fmaf(q, |s_axis|, org_axis),
+// where q is byte c (child c) of the plane's word and s_axis = +-2^k the
+// axis's scale, stored as a float so that the traversal scales it by the
+// ray's reciprocal direction in one multiply (the sign goes as a source
+// modifier). The build rounds lower planes down and upper planes up through
+// exactly this expression, so every quantised box contains the child's exact
+// box. The signs of s_y and s_z hold nchild - 1 (bit 0 and bit 1). Children
+// 0..nchild-1 are valid; slots beyond hold an inverted box (low bytes 255,
+// high bytes 0) and a leaf of the node's subtree. child[c] >= 0 is a
 // four-wide node index, child[c] < 0 is leaf ~prim. 64 bytes.
 struct __align__(16) bvh4_node {
-  float ox, oy, oz;
-  uint32_t meta;  // e_x | e_y << 8 | e_z << 16 | nchild << 24
+  float ox, oy, oz, sx;
   uint32_t lo_x, lo_y, lo_z, hi_x;
-  uint32_t hi_y, hi_z, pad0, pad1;
+  uint32_t hi_y, hi_z;
+  float sy, sz;
   int4 child;
 };
+
+__host__ __device__ inline int bvh4_nchild(const bvh4_node& nd) {
+  return 1 + (nd.sy < 0.0f ? 1 : 0) + (nd.sz < 0.0f ? 2 : 0);
+}
 static_assert(sizeof(bvh4_node) == 64, "four-wide node must be one 64-byte record");
 
 // Per-instance description, read by each lane for the instance its ray

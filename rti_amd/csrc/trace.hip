@@ -228,10 +228,9 @@ __device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& 
   H.ch = reinterpret_cast<const int4*>(q)[3];
   if (COUNT) C.nodes++;
   const f3 inv = T.inv, oi = T.oi;
-  const uint32_t meta = q0.w;
-  const float ax = __uint_as_float((meta & 0xffu) << 23) * inv.x;
-  const float ay = __uint_as_float(((meta >> 8) & 0xffu) << 23) * inv.y;
-  const float az = __uint_as_float(((meta >> 16) & 0xffu) << 23) * inv.z;
+  const float ax = fabsf(__uint_as_float(q0.w)) * inv.x;
+  const float ay = fabsf(__uint_as_float(q2.z)) * inv.y;
+  const float az = fabsf(__uint_as_float(q2.w)) * inv.z;
   const float bx = fmaf(__uint_as_float(q0.x), inv.x, -oi.x);
   const float by = fmaf(__uint_as_float(q0.y), inv.y, -oi.y);
   const float bz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
@@ -383,7 +382,7 @@ __device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uin
     for (int c = 0; c < 4; ++c) {
       if (leaves & (1u << c)) {
         qa -= step;
-        *lds_word(qa) = ~sel4(H.ch, (uint32_t)c);
+        *lds_word(qa) = sel4(H.ch, (uint32_t)c);  // ~prim
       }
     }
     T.qa = qa;
@@ -395,7 +394,7 @@ __device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uin
 // Leaf phase: the most recently queued leaf of this lane.
 template <int LS, bool COUNT>
 __device__ __forceinline__ void leaf_visit(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
-  const uint32_t p = *lds_word(T.qa);
+  const uint32_t p = ~*lds_word(T.qa);
   T.qa += P.block * 4u;
   test_prim<COUNT>(P, o, d, p, T, C);
 }

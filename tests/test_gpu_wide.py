@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from rti_amd import Context, geometry, run_scene
+from rti_amd._lib import wide_scales
 from tests.helpers import small_hole_mesh, small_trench
 
 pytestmark = pytest.mark.gpu
@@ -32,9 +33,10 @@ def _decode(node, c):
     """Child c's box planes as the traversal computes them (float32)."""
     lo = np.zeros(3)
     hi = np.zeros(3)
+    scales, _ = wide_scales(node)
     for k in range(3):
-        e = (int(node["meta"]) >> (8 * k)) & 0xFF
-        s = np.float32(np.ldexp(1.0, e - 127))
+        s = scales[k]
+        assert s == np.float32(np.ldexp(1.0, int(np.frexp(s)[1]) - 1)), "scales are powers of two"
         ql = (int(node["lo"][k]) >> (8 * c)) & 0xFF
         qh = (int(node["hi"][k]) >> (8 * c)) & 0xFF
         # q * s is exact in float32; one rounding for the sum, as fmaf does
@@ -53,8 +55,12 @@ def _validate_wide(wide, perm, scene):
         v = stack.pop()
         seen_node[v] += 1
         order.append(v)
-        m = int(wide["meta"][v]) >> 24
-        assert 1 <= m <= 4, f"node {v} has {m} children"
+        _, m = wide_scales(wide[v])
+        for c in range(m, 4):
+            # empty slot: inverted box, and a leaf (primitive) as the child
+            assert all((int(wide["lo"][v][k]) >> (8 * c)) & 0xFF == 0xFF for k in range(3))
+            assert all((int(wide["hi"][v][k]) >> (8 * c)) & 0xFF == 0 for k in range(3))
+            assert int(wide["child"][v][c]) < 0
         for c in range(m):
             ch = int(wide["child"][v][c])
             if ch >= 0:
@@ -67,7 +73,7 @@ def _validate_wide(wide, perm, scene):
     sub_hi = np.full((len(wide), 3), -np.inf)
     bad = 0
     for v in reversed(order):
-        m = int(wide["meta"][v]) >> 24
+        _, m = wide_scales(wide[v])
         for c in range(m):
             ch = int(wide["child"][v][c])
             blo, bhi = (plo[~ch], phi[~ch]) if ch < 0 else (sub_lo[ch], sub_hi[ch])
@@ -78,6 +84,10 @@ def _validate_wide(wide, perm, scene):
                     print("node", v, "child", c, "->", ch, "box", qlo, qhi, "needs", blo, bhi)
             sub_lo[v] = np.minimum(sub_lo[v], blo)
             sub_hi[v] = np.maximum(sub_hi[v], bhi)
+        for c in range(m, 4):
+            p = ~int(wide["child"][v][c])
+            assert np.all(plo[p] >= sub_lo[v]) and np.all(phi[p] <= sub_hi[v]), \
+                f"node {v}: the empty slot's leaf {p} lies outside the subtree"
     assert bad == 0, f"{bad} quantised child boxes do not enclose their subtree"
 
 
