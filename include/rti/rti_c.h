@@ -119,7 +119,7 @@ typedef struct rti_params {
   int32_t leaf_batch;     /* persistent kernel, four-wide tree: leaf tests are queued per
                              lane and run in wave-wide leaf phases once this many 64ths
                              of the wave's tracing lanes hold a queued leaf (1..64);
-                             0 = default (64 when the store holds triangles, else -1);
+                             0 = default (48 when the store holds triangles, else -1);
                              -1 = test leaves at each node visit                       */
   int32_t tree_builder;   /* rti_tree_builder; 0 = default (PLOC for triangle meshes,
                              LBVH for disc clouds)                                      */

@@ -928,7 +928,7 @@ int default_compact_lanes() { return 56; }
 // Postponed leaves pay off where a leaf test is expensive: triangle stores
 // (measured +5 % on a 199k-triangle hole) but not stores of discs only, whose
 // leaf test is a few instructions (batch of disc clouds -5 %, rough surface -3 %).
-int default_leaf_batch(bool triangles) { return triangles ? 64 : 0; }
+int default_leaf_batch(bool triangles) { return triangles ? 48 : 0; }
 int wide_stack_capacity() { return kWideStack; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {

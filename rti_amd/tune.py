@@ -7,7 +7,8 @@ closest-hit query before the wave shades and refills), then the resident
 waves per CU (0 = as many as registers and LDS allow; fewer can suit
 workloads whose working set outgrows the caches), then the LDS stack depth,
 then the block size, then ``leaf_batch`` (-1: leaf tests at each node visit;
-64: postponed leaf tests in wave-wide leaf phases), then ``refill_lanes``
+32, 48, 64: postponed leaf tests in wave-wide leaf phases, started once that
+many 64ths of the tracing lanes hold a queued leaf), then ``refill_lanes``
 (lanes of a wave that must wait for a new ray before the wave refills). Results are cached in a JSON file keyed by the
 batch signature, so a repeated workload is not re-tuned.
 """
@@ -20,7 +21,7 @@ from ._lib import Context
 COMPACT = (40, 48, 56, 60, 64)
 LDS = (4, 8, 16)
 BLOCK = (128, 256)
-LEAF = (-1, 64)
+LEAF = (-1, 32, 48, 64)
 REFILL = (1, 8, 16)
 WAVES = (0, 20, 24)
 
