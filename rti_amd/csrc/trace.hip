@@ -172,7 +172,10 @@ __device__ __forceinline__ bool trav_step(const tparams& P, const f3 o, const f3
     T.cur = ch.y;
     return false;
   }
-  if (T.sp == 0) return true;
+  if (T.sp == 0) {
+    T.cur = -1;  // finished: the persistent loop ballots T.cur >= 0
+    return true;
+  }
   --T.sp;
   T.cur = (int)(T.sp < LS ? lds[__umul24((uint32_t)T.sp, stride)] : ovf[T.sp - LS]);
   return false;
@@ -344,7 +347,9 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
     test_prim<COUNT>(P, o, d, ~sel4(H.ch, c), T, C);
   }
   if (order_push<LS, true>(P, T, H, ovf)) return false;
-  return !pop4<LS>(P, T, ovf);
+  if (pop4<LS>(P, T, ovf)) return false;
+  T.cur = -1;  // finished: the persistent loop ballots T.cur >= 0
+  return true;
 }
 
 // Postponed leaf tests. A wave tests leaves only in leaf phases, where every
@@ -822,11 +827,16 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     // so a few long paths still advance many nodes per shading pass.
     const int still = ((64 - compact) * __popcll(live)) >> 6;
     if constexpr (PL) {
+      // tr mirrors the lanes' tracing flags as a wave mask. It is kept up to
+      // date from ballots of the plain comparisons (each one compare into a
+      // scalar mask, shared with the lanes' own branch conditions) rather
+      // than re-balloted from the flags, whose ballot would first turn the
+      // flag into a 0/1 register and compare that.
+      unsigned long long tr = __ballot(tracing);
       for (;;) {
-        const unsigned long long tr = __ballot(tracing);
-        const unsigned long long queued = __ballot(tracing && T.qa != qtop);
-        const unsigned long long full = __ballot(tracing && T.qa < qfull);
-        const unsigned long long nodes = __ballot(tracing && T.cur >= 0);
+        const unsigned long long queued = tr & __ballot(T.qa != qtop);
+        const unsigned long long full = tr & __ballot(T.qa < qfull);
+        const unsigned long long nodes = tr & __ballot(T.cur >= 0);
         const bool leaf_phase =
             queued != 0ull && (full != 0ull || nodes == 0ull || __popcll(queued) * 64 >= P.leaf_min * __popcll(tr));
         if (leaf_phase) {
@@ -834,13 +844,20 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
         } else {
           if (tracing && T.cur >= 0) S.template visit<COUNT>(P, T, C);
         }
-        tracing = tracing && (T.cur >= 0 || T.qa != qtop);
-        if (__popcll(__ballot(tracing)) <= still) break;
+        const bool more = T.cur >= 0 || T.qa != qtop;
+        tr &= __ballot(T.cur >= 0) | __ballot(T.qa != qtop);
+        tracing = tracing && more;
+        if (__popcll(tr) <= still) break;
       }
     } else {
+      // a finished query leaves T.cur = -1, so the wave mask of tracing
+      // lanes follows from one compare (see the postponed-leaf loop above)
+      unsigned long long tr = __ballot(tracing);
       for (;;) {
-        if (tracing) tracing = !S.template step<COUNT>(P, R.o, R.d, T, C);
-        if (__popcll(__ballot(tracing)) <= still) break;
+        if (tracing) S.template step<COUNT>(P, R.o, R.d, T, C);
+        tr &= __ballot(T.cur >= 0);
+        tracing = tracing && T.cur >= 0;
+        if (__popcll(tr) <= still) break;
       }
     }
     if (alive && !tracing) {
