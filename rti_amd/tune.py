@@ -2,7 +2,7 @@
 
 For a batch (the same item dicts ``Context.batch_set`` takes) it traces a
 reduced ray budget with each candidate and keeps the fastest, by coordinate
-descent: ``compact_lanes`` (lanes of a wave that must have finished their
+descent (block size and leaf threshold visited twice): ``compact_lanes`` (lanes of a wave that must have finished their
 closest-hit query before the wave shades and refills), then the resident
 waves per CU (0 = as many as registers and LDS allow; fewer can suit
 workloads whose working set outgrows the caches), then the LDS stack depth,
@@ -44,11 +44,13 @@ def _rate(ctx, knobs):
     return float(ctx.counters()[0]) / (st["trace_ms"] / 1e3)
 
 
-def default_fraction(items, sample_rays=5e7, full_up_to=1e8):
+def default_fraction(items, sample_rays=2e8, full_up_to=2e8):
     """Fraction of the rays the candidates are timed on: all of them up to
     ``full_up_to`` rays (a small sample of a batch is dominated by its single
     longest path, which does not shrink with the sample), else about
-    ``sample_rays``."""
+    ``sample_rays``. Smaller samples rank close candidates differently from
+    the full launch (on the 6e8-ray hole a 5e7-ray sample preferred a
+    configuration 1.5 % slower at full size)."""
     total = float(sum(it["n_rays"] for it in items))
     return 1.0 if total <= full_up_to else sample_rays / total
 
@@ -70,8 +72,11 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     with Context() as ctx:
         ctx.batch_set(small)
         ctx.prepare()
+        # one pass over all knobs, then the block size and the leaf-phase
+        # threshold once more (the two interact most)
         for name, values in (("compact_lanes", COMPACT), ("waves_per_cu", WAVES), ("lds_stack", LDS),
-                             ("block_size", BLOCK), ("leaf_batch", LEAF), ("refill_lanes", REFILL)):
+                             ("block_size", BLOCK), ("leaf_batch", LEAF), ("refill_lanes", REFILL),
+                             ("block_size", BLOCK), ("leaf_batch", LEAF)):
             scores = {}
             for v in values:
                 knobs = dict(best, **{name: v})
