@@ -18,7 +18,7 @@ import os
 
 from ._lib import Context
 
-COMPACT = (40, 48, 56, 60, 64)
+COMPACT = (40, 48, 52, 56, 60, 64)
 LDS = (4, 8, 16)
 BLOCK = (128, 256)
 LEAF = (-1, 32, 48, 64)
