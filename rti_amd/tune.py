@@ -24,6 +24,7 @@ BLOCK = (128, 256)
 LEAF = (-1, 32, 48, 64)
 REFILL = (1, 8, 16)
 WAVES = (0, 20, 24)
+MIN_GAIN = 0.005  # relative gain a coordinate step must show to change a knob
 
 
 def signature(items):
@@ -84,7 +85,13 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
                 trials.append({**knobs, "segments_per_s": scores[v]})
                 if log:
                     log(f"tune {knobs} -> {scores[v]:.3e} segments/s")
-            best[name] = max(scores, key=scores.get)
+            # move only for a clear gain: candidates within the replay-to-replay
+            # spread would otherwise steer the descent by noise (waves_per_cu
+            # 24 at +0.1 % once hid lds_stack 4, +1.8 % with waves_per_cu 0)
+            top = max(scores, key=scores.get)
+            cur = scores.get(best[name])  # None: the starting value is not a candidate
+            if cur is None or scores[top] > cur * (1.0 + MIN_GAIN):
+                best[name] = top
     out = dict(best, segments_per_s=max(t["segments_per_s"] for t in trials), trials=trials, signature=key)
     if cache:
         db = {}
