@@ -356,18 +356,26 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
 // lane with a queued leaf tests one, instead of after each node visit, where
 // a node visit's leaves keep the whole wave while a few lanes test them.
 // Queued leaf j of thread t: the LDS word after the lane stacks at
-// j * block + t. A node visit queues at most four leaves.
-constexpr uint32_t kLeafQueue = 8;
+// j * block + t. A node visit queues at most four leaves. The queue holds 12
+// leaves beside a 4-entry lane stack and 8 beside deeper ones: 80 and 96
+// bytes per thread, so that a 256-thread block takes 20 and 24 KB and the
+// four-entry configuration keeps all 28 waves a CU's registers allow
+// (12 instead of 8 entries: hole +1.1 %, fewer phases forced by a full queue).
+template <int LS>
+constexpr uint32_t leaf_queue_len() {
+  return LS <= 4 ? 12u : 8u;
+}
 
 // Ray ids a persistent wave reserves beyond its lanes' needs.
 constexpr unsigned long long kRayReserve = 256;
 
 // The lane's queue is addressed by a running LDS byte address (T.qa) and
 // grows downwards, one slot (block * 4 bytes) per entry, from its empty top
-// at slot kLeafQueue: queueing a leaf costs one subtract beside the write.
+// at slot leaf_queue_len: queueing a leaf costs one subtract beside the write.
 template <int LS>
 __device__ __forceinline__ uint32_t leaf_queue_top(uint32_t stride) {
-  return (uint32_t)(size_t)((lds_u32*)(trace_lds) + (2u * LS * stride + kLeafQueue * stride + threadIdx.x));
+  return (uint32_t)(size_t)((lds_u32*)(trace_lds) +
+                            (2u * LS * stride + leaf_queue_len<LS>() * stride + threadIdx.x));
 }
 
 __device__ __forceinline__ lds_u32* lds_word(uint32_t byte_addr) { return (lds_u32*)(size_t)byte_addr; }
@@ -783,7 +791,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   // the lane's leaf queue: empty at qtop, no room for another node's four
   // leaves below qfull
   const uint32_t qtop = PL ? leaf_queue_top<LS>(P.block) : 0u;
-  const uint32_t qfull = qtop - (kLeafQueue - 4) * P.block * 4u;
+  const uint32_t qfull = qtop - (leaf_queue_len<LS>() - 4) * P.block * 4u;
   for (;;) {
     // ended rays are replaced once refill_min lanes wait for one (a refill
     // costs the wave the same whatever the number of lanes it serves), or
@@ -934,7 +942,8 @@ kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool a
 // Dynamic LDS: the lane stacks, then (postponed leaves) the leaf queues.
 size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) {
   size_t b = (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
-  if (width == 4 && postpone) b += (size_t)block_size * kLeafQueue * 4;
+  if (width == 4 && postpone)
+    b += (size_t)block_size * (lds_stack <= 4 ? leaf_queue_len<4>() : leaf_queue_len<8>()) * 4;
   return b;
 }
 
