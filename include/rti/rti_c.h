@@ -106,7 +106,7 @@ typedef struct rti_params {
   int32_t adjoint;        /* 1 = also accumulate d/ds and d/dn                          */
   int32_t accum;          /* rti_accum                                                  */
   int32_t block_size;     /* 0 = tuned default; else 64, 128, 192 or 256                */
-  int32_t lds_stack;      /* 0 = tuned default; else 4, 8, 16, 24 or 32 LDS stack slots */
+  int32_t lds_stack;      /* 0 = tuned default; else 4, 6, 8, 16, 24 or 32 LDS stack slots */
   int32_t waves_per_cu;   /* persistent kernel: resident waves per CU, 0 = default     */
   int32_t compact_lanes;  /* persistent kernel: lanes of a wave whose closest-hit query
                              must be finished before the wave shades and refills

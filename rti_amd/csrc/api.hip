@@ -148,9 +148,9 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.accum != RTI_ACCUM_FIXED64 && p.accum != RTI_ACCUM_FP64) return fail(ctx, RTI_ERR_INVALID_ARG, "bad accum");
   if (p.block_size != 0 && (p.block_size < 64 || p.block_size > 256 || (p.block_size & 63)))
     return fail(ctx, RTI_ERR_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
-  if (p.lds_stack != 0 && p.lds_stack != 4 && p.lds_stack != 8 && p.lds_stack != 16 && p.lds_stack != 24 &&
-      p.lds_stack != 32)
-    return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 8, 16, 24 or 32");
+  if (p.lds_stack != 0 && p.lds_stack != 4 && p.lds_stack != 6 && p.lds_stack != 8 && p.lds_stack != 16 &&
+      p.lds_stack != 24 && p.lds_stack != 32)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 6, 8, 16, 24 or 32");
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
   if (p.leaf_batch < -1 || p.leaf_batch > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..64");

@@ -357,10 +357,11 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
 // a node visit's leaves keep the whole wave while a few lanes test them.
 // Queued leaf j of thread t: the LDS word after the lane stacks at
 // j * block + t. A node visit queues at most four leaves. The queue holds 12
-// leaves beside a 4-entry lane stack and 8 beside deeper ones: 80 and 96
-// bytes per thread, so that a 256-thread block takes 20 and 24 KB and the
-// four-entry configuration keeps all 28 waves a CU's registers allow
-// (12 instead of 8 entries: hole +1.1 %, fewer phases forced by a full queue).
+// leaves beside a 4-entry lane stack and 8 beside deeper ones: 80 bytes per
+// thread with 4 or 6 stack entries, which leaves LDS for all 28 waves a CU's
+// registers allow, 96 with 8 (24 waves at 256 threads per block); 12 instead
+// of 8 entries measured +1.1 % on the hole (fewer phases forced by a full
+// queue), a 6-entry stack with 8 queue entries another +1.8 %.
 template <int LS>
 constexpr uint32_t leaf_queue_len() {
   return LS <= 4 ? 12u : 8u;
@@ -925,6 +926,7 @@ template <int W>
 kfn pick_ls(int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count) {
   switch (lds_stack) {
     case 4: return pick<4, W>(persistent, postpone, adj, fp64, count);
+    case 6: return pick<6, W>(persistent, postpone, adj, fp64, count);
     case 8: return pick<8, W>(persistent, postpone, adj, fp64, count);
     case 16: return pick<16, W>(persistent, postpone, adj, fp64, count);
     case 24: return pick<24, W>(persistent, postpone, adj, fp64, count);

@@ -19,7 +19,7 @@ import os
 from ._lib import Context
 
 COMPACT = (40, 48, 52, 56, 60, 64)
-LDS = (4, 8, 16)
+LDS = (4, 6, 8, 16)
 BLOCK = (128, 256)
 LEAF = (-1, 32, 48, 64)
 REFILL = (1, 8, 16)

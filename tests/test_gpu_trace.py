@@ -183,7 +183,8 @@ def test_bit_exact_rerun_and_launch_shapes():
     c = run_scene(sc, kernel=1, **kw)
     d = run_scene(sc, block_size=64, lds_stack=4, **kw)
     e = run_scene(sc, block_size=128, lds_stack=32, **kw)
-    for other in (b, c, d, e):
+    f = run_scene(sc, block_size=192, lds_stack=6, **kw)
+    for other in (b, c, d, e, f):
         assert np.array_equal(a["raw"], other["raw"])
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["smoothed"], other["smoothed"])

@@ -148,10 +148,11 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     d = run_scene(sc, bvh_width=4, leaf_batch=-1, **base)
     e = run_scene(sc, bvh_width=4, leaf_batch=1, **base)
     f = run_scene(sc, bvh_width=4, leaf_batch=64, block_size=64, lds_stack=4, **base)
+    f6 = run_scene(sc, bvh_width=4, leaf_batch=48, block_size=128, lds_stack=6, **base)
     # the other binary-tree topology: the closest hit does not depend on the tree
     g = run_scene(sc, tree_builder=1, **base)
     h = run_scene(sc, tree_builder=2, **base)
-    for other in (b, c, d, e, f, g, h):
+    for other in (b, c, d, e, f, f6, g, h):
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["raw"], other["raw"])
         assert a["stats"]["segments"] == other["stats"]["segments"]
