@@ -141,6 +141,7 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     base = dict(n_rays=60_000, seed=11, **kw)
     a = run_scene(sc, bvh_width=4, **base)
     b = run_scene(sc, bvh_width=2, **base)
+    b6 = run_scene(sc, bvh_width=2, lds_stack=6, block_size=192, **base)
     c = run_scene(sc, bvh_width=4, kernel=1, **base)
     # leaf tests at each node visit, and postponed leaf tests with a leaf
     # phase as soon as one lane queues a leaf / only when every tracing lane
@@ -152,7 +153,7 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     # the other binary-tree topology: the closest hit does not depend on the tree
     g = run_scene(sc, tree_builder=1, **base)
     h = run_scene(sc, tree_builder=2, **base)
-    for other in (b, c, d, e, f, f6, g, h):
+    for other in (b, b6, c, d, e, f, f6, g, h):
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["raw"], other["raw"])
         assert a["stats"]["segments"] == other["stats"]["segments"]
