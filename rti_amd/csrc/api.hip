@@ -509,9 +509,9 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
   const rti_params& p = ctx->params;
   cfg.block = p.block_size ? p.block_size : 256;
   cfg.width = p.bvh_width ? p.bvh_width : 4;
-  cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width);
   cfg.leaf = p.leaf_batch < 0 ? 0
              : (p.leaf_batch ? p.leaf_batch : rti_impl::default_leaf_batch(ctx->prim_shift == rti_dev::kPrimShiftMixed));
+  cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width, cfg.width == 4 && cfg.leaf > 0);
   if (cfg.width == 4) {
     for (auto& I : ctx->inst)
       if ((int)I.wide_stack > rti_impl::wide_stack_capacity())

@@ -111,7 +111,7 @@ struct trace_launch {
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
 hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,
                            int width, int* blocks_per_cu);
-int default_lds_stack(int width);
+int default_lds_stack(int width, bool postpone);
 // Stack entries (LDS + private) a four-wide traversal holds at most.
 int wide_stack_capacity();
 int default_compact_lanes();

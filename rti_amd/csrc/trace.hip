@@ -951,7 +951,9 @@ size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) 
 
 }  // namespace
 
-int default_lds_stack(int width) { return width == 4 ? 8 : 16; }
+// Six entries beside a leaf queue keep the block's LDS within what all the
+// register-limited waves need; eight without one.
+int default_lds_stack(int width, bool postpone) { return width == 4 ? (postpone ? 6 : 8) : 16; }
 int default_compact_lanes() { return 56; }
 // Postponed leaves pay off where a leaf test is expensive: triangle stores
 // (measured +5 % on a 199k-triangle hole) but not stores of discs only, whose
