@@ -217,3 +217,46 @@ def test_radiosity_direct_flux_quadrature_matches_closed_form():
     t = radiosity.solve_trench(1.0, 4.0, 0.1, n_wall=40, n_floor=10)
     qt = radiosity.trench_direct_flux(t["strips"], 1.0, 4.0, 1.0)
     assert np.max(np.abs(qt - t["G0"]) / t["G0"]) < 2e-3
+
+
+def test_tuner_descent_ignores_gains_within_noise(monkeypatch):
+    """rti_amd.tune moves a knob only for a gain above MIN_GAIN: a 0.1 %
+    'gain' of waves_per_cu = 24 (replay noise) must not hide the real +1.8 %
+    of lds_stack = 4 that only holds with waves_per_cu = 0. The kernel is
+    replaced by a rate model, so no GPU is used."""
+    from rti_amd import tune
+
+    class FakeContext:
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def batch_set(self, items):
+            pass
+
+        def prepare(self):
+            pass
+
+    def rate(ctx, knobs):
+        r = 5.4e9
+        if knobs["waves_per_cu"] == 24:
+            r *= 1.001
+        if knobs["lds_stack"] == 4 and knobs["waves_per_cu"] == 0:
+            r *= 1.018
+        if knobs["leaf_batch"] == 48:
+            r *= 1.01
+        return r
+
+    monkeypatch.setattr(tune, "Context", FakeContext)
+    monkeypatch.setattr(tune, "_rate", rate)
+    scene = {"kind": "triangles", "tris": np.zeros((4, 3), np.uint32), "verts": np.zeros((3, 3), np.float32)}
+    out = tune.tune([{"scene": scene, "n_rays": 1_000_000}], fraction=1.0)
+    k = tune.knobs(out)
+    assert k["waves_per_cu"] == 0
+    assert k["lds_stack"] == 4
+    assert k["leaf_batch"] == 48
+    # every knob was visited: six knobs, block size and leaf threshold twice
+    assert len(out["trials"]) == (len(tune.COMPACT) + len(tune.WAVES) + len(tune.LDS) + 2 * len(tune.BLOCK)
+                                  + 2 * len(tune.LEAF) + len(tune.REFILL))
