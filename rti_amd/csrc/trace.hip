@@ -457,19 +457,21 @@ __device__ __forceinline__ void face_test(int f, float t, float a, float alo, fl
   }
 }
 
-__device__ __forceinline__ int boundary_hit(const inst_desc& D, f3 o, f3 d, float tnear, float& tb) {
+// rx, ry: hardware reciprocals (1 ulp) of d.x and d.y, as for the slab tests.
+__device__ __forceinline__ int boundary_hit(const inst_desc& D, f3 o, f3 d, float rx, float ry, float tnear,
+                                            float& tb) {
   int face = -1;
   float best = FLT_MAX;
   if (d.x != 0.0f) {
-    float t = (D.xmax - o.x) / d.x;
+    float t = (D.xmax - o.x) * rx;
     face_test(0, t, fmaf(d.y, t, o.y), D.ymin, D.ymax, fmaf(d.z, t, o.z), D, tnear, best, face);
-    t = (D.xmin - o.x) / d.x;
+    t = (D.xmin - o.x) * rx;
     face_test(2, t, fmaf(d.y, t, o.y), D.ymin, D.ymax, fmaf(d.z, t, o.z), D, tnear, best, face);
   }
   if (d.y != 0.0f) {
-    float t = (D.ymin - o.y) / d.y;
+    float t = (D.ymin - o.y) * ry;
     face_test(1, t, fmaf(d.x, t, o.x), D.xmin, D.xmax, fmaf(d.z, t, o.z), D, tnear, best, face);
-    t = (D.ymax - o.y) / d.y;
+    t = (D.ymax - o.y) * ry;
     face_test(3, t, fmaf(d.x, t, o.x), D.xmin, D.xmax, fmaf(d.z, t, o.z), D, tnear, best, face);
   }
   tb = best;
@@ -616,8 +618,17 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   const inst_desc& D = P.desc[R.inst];
   C.seg++;
   R.segs++;
+  const float tiny = 1e-20f;
+  // slab factors only cull boxes (every box is padded beyond its
+  // primitives), so the hardware reciprocal (1 ulp) suffices; the
+  // boundary-face distances multiply by it too (Embree's own intersectors use
+  // a refined reciprocal, not a correctly rounded division, so neither form
+  // is the reference's bits)
+  T.inv.x = __builtin_amdgcn_rcpf(fabsf(R.d.x) > tiny ? R.d.x : copysignf(tiny, R.d.x));
+  T.inv.y = __builtin_amdgcn_rcpf(fabsf(R.d.y) > tiny ? R.d.y : copysignf(tiny, R.d.y));
+  T.inv.z = __builtin_amdgcn_rcpf(fabsf(R.d.z) > tiny ? R.d.z : copysignf(tiny, R.d.z));
   T.tnear = D.tnear;
-  T.face = boundary_hit(D, R.o, R.d, T.tnear, T.tb);
+  T.face = boundary_hit(D, R.o, R.d, T.inv.x, T.inv.y, T.tnear, T.tb);
   T.tbest = T.tb;
   T.found = false;
   T.pbest = 0;
@@ -625,12 +636,6 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.curd = 0.0f;
   T.sp = 0;
   T.qa = qtop;
-  const float tiny = 1e-20f;
-  // slab factors only cull boxes (every box is padded beyond its
-  // primitives), so the hardware reciprocal (1 ulp) suffices
-  T.inv.x = __builtin_amdgcn_rcpf(fabsf(R.d.x) > tiny ? R.d.x : copysignf(tiny, R.d.x));
-  T.inv.y = __builtin_amdgcn_rcpf(fabsf(R.d.y) > tiny ? R.d.y : copysignf(tiny, R.d.y));
-  T.inv.z = __builtin_amdgcn_rcpf(fabsf(R.d.z) > tiny ? R.d.z : copysignf(tiny, R.d.z));
   T.oi = make(R.o.x * T.inv.x, R.o.y * T.inv.y, R.o.z * T.inv.z);
 }
 
