@@ -177,6 +177,23 @@ int rti_prepare(rti_ctx* ctx);
 int rti_trace(rti_ctx* ctx);
 int rti_finalize(rti_ctx* ctx);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the trace of rti_trace on a context already built by
+ * rti_prepare, calling no hipMalloc / hipFree. The accumulator records go to
+ * the caller's device buffer `acc` of acc_words >= 4 x (primitives of all
+ * instances) 64-bit words, cleared by the call: per primitive, in stored
+ * order (rti_get_hierarchy's perm maps a stored position to its input id),
+ * {sum of deposits, sum of squared deposits, deposit count, 0}, the sums in
+ * fixed point with scale 2^fixed_shift (rti_get_stats). The instance
+ * descriptors, ray queue and counters live in a static per-device array of
+ * 33,024 bytes, which holds at most 256 instances (more: RTI_ERR_INVALID_ARG);
+ * every call on one GPU shares it, so calls that overlap on one GPU race.
+ * params.adjoint and params.moments are rejected (RTI_ERR_INVALID_ARG); an
+ * unbuilt context returns RTI_ERR_STATE. rti_get_counters reports the trace;
+ * the context's own results and rti_finalize need a new rti_trace/rti_run. */
+int rti_trace_noalloc(rti_ctx* ctx, unsigned long long* acc, uint64_t acc_words);
+
 /* Any output pointer may be NULL. smoothed_norm: reference mc estimates
  * (neighbourhood mean of value/area, divided by the max); raw: sum of
  * deposits per primitive (reference `value`); sigma: Monte Carlo standard

@@ -39,6 +39,7 @@ EXPORTS = [
     "rti_get_hierarchy", "rti_get_wide_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
     "rti_batch_get_stats", "rti_get_error_stats", "rti_batch_get_error_stats", "rti_batch_get_adjoint",
+    "rti_trace_noalloc",
 ]
 
 
@@ -149,6 +150,7 @@ def _declare(L):
     L.rti_get_error_stats.argtypes = [vp, _dp, _dp]
     L.rti_batch_get_error_stats.argtypes = [vp, c.c_uint32, _dp, _dp]
     L.rti_batch_get_adjoint.argtypes = [vp, c.c_uint32, _dp, _dp]
+    L.rti_trace_noalloc.argtypes = [vp, c.c_void_p, c.c_uint64]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("rti_default_params", "rti_last_error"):

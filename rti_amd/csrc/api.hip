@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,7 @@ struct rti_ctx {
   bool dirty = true;        // geometry changed since the last prepare
   bool batch_mode = false;  // instances came from rti_batch_set (own physics parameters)
   bool have_results = false;
+  bool acc_traced = false;  // the context's accumulators hold a trace of the current store
   // scene store
   dbuf prims, perm, nodes, nodes4, nb_ptr, nb_col, areas, desc;
   uint32_t prim_shift = rti_dev::kPrimShiftMixed;  // log2 float4 per primitive record
@@ -114,6 +116,43 @@ struct rti_ctx {
 };
 
 namespace {
+
+// ---- static scratch of rti_trace_noalloc ----
+// The no-allocation trace keeps its instance descriptors, ray queue and
+// counters in this file-scope device array: one copy per GPU, held until the
+// process ends, shared by every such call on that GPU (calls that overlap on
+// one GPU race on it). Layout: [0, 8) ray queue, [64, 128) the eight
+// counters, [256, ...) up to kNoAllocMaxInstances descriptors. In host code
+// the name `scratch` is a host-side stand-in, not a device address; host code
+// uses only the per-device address resolved by resolve_scratch.
+constexpr uint32_t kNoAllocMaxInstances = 256;
+constexpr size_t kScratchStatsOffset = 64;
+constexpr size_t kScratchDescOffset = 256;
+constexpr size_t kScratchBytes = kScratchDescOffset + kNoAllocMaxInstances * sizeof(inst_desc);  // 33,024 B
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device
+
+// Resolves the device address of `scratch` for `device`, once per device;
+// the caller has made `device` current.
+hipError_t resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 int fail(rti_ctx* ctx, int code, const char* fmt, ...) {
   char buf[512];
@@ -428,6 +467,7 @@ int prepare(rti_ctx* ctx) {
   (void)hipEventDestroy(b1);
   ctx->build_ms = ms;
   ctx->dirty = false;
+  ctx->acc_traced = false;
   return RTI_OK;
 }
 
@@ -444,7 +484,7 @@ double path_cost(const rti_params& p) {
 // decreasing expected path length, so the longest paths start first and the
 // launch does not end on a tail of long paths. The order does not change any
 // result: an instance's paths depend on its own key and ray ids only.
-int upload_descriptors(rti_ctx* ctx) {
+int upload_descriptors(rti_ctx* ctx, inst_desc* dst) {
   const size_t n = ctx->inst.size();
   std::vector<size_t> order(n);
   for (size_t k = 0; k < n; ++k) order[k] = k;
@@ -495,8 +535,7 @@ int upload_descriptors(rti_ctx* ctx) {
     rb += p.n_rays;
   }
   ctx->rays_total = rb;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->desc.p, D.data(), D.size() * sizeof(inst_desc), hipMemcpyHostToDevice,
-                              ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(dst, D.data(), D.size() * sizeof(inst_desc), hipMemcpyHostToDevice, ctx->stream));
   return RTI_OK;
 }
 
@@ -533,29 +572,46 @@ int alloc_accumulators(rti_ctx* ctx, bool adj) {
   return RTI_OK;
 }
 
+// Device memory one trace writes besides the scene store: the accumulator
+// records, the instance descriptors, the ray queue and the counters (the
+// context's own buffers, or the caller's records and the static scratch of
+// the no-allocation entry point).
+struct trace_bufs {
+  unsigned long long* acc;
+  inst_desc* desc;
+  unsigned long long* queue;
+  unsigned long long* stats;  // 8 counters
+};
+
+trace_bufs ctx_bufs(rti_ctx* ctx) {
+  return {ctx->acc.as<unsigned long long>(), ctx->desc.as<inst_desc>(), ctx->queue.as<unsigned long long>(),
+          ctx->stats.as<unsigned long long>()};
+}
+
 // Clears accumulators / queue / counters and launches the trace on the stream.
 // `only` selects one instance for the plain per-geometry path (-1: all).
-int enqueue_trace(rti_ctx* ctx, const kernel_cfg& cfg, bool adj, bool fp64, bool count, int kernel, long only) {
+int enqueue_trace(rti_ctx* ctx, const trace_bufs& B, const kernel_cfg& cfg, bool adj, bool fp64, bool count,
+                  int kernel, long only) {
   const uint64_t np = ctx->n_total;
-  HIP_TRY(ctx, rti_impl::fill_u64(ctx->acc.as<unsigned long long>(), np * rti_dev::kAccStride, 0ull, ctx->stream));
+  HIP_TRY(ctx, rti_impl::fill_u64(B.acc, np * rti_dev::kAccStride, 0ull, ctx->stream));
   if (adj) HIP_TRY(ctx, rti_impl::fill_u64(ctx->adj.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
   const bool mom = ctx->params.moments != 0;
   if (mom)  // 0.0 has all bits zero
     HIP_TRY(ctx, rti_impl::fill_u64(ctx->mom.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
-  HIP_TRY(ctx, rti_impl::fill_u64(ctx->stats.as<unsigned long long>(), 8, 0ull, ctx->stream));
+  HIP_TRY(ctx, rti_impl::fill_u64(B.stats, 8, 0ull, ctx->stream));
   rti_impl::trace_launch L{};
-  L.desc = ctx->desc.as<inst_desc>();
+  L.desc = B.desc;
   L.n_inst = (uint32_t)ctx->inst.size();
   L.prims = ctx->prims.as<float4>();
   L.prim_shift = ctx->prim_shift;
   L.nodes = ctx->nodes.as<bvh_node>();
   L.nodes4 = ctx->nodes4.as<bvh4_node>();
   L.nb_col = ctx->nb_col.as<uint32_t>();
-  L.acc = ctx->acc.as<unsigned long long>();
+  L.acc = B.acc;
   L.adj = adj ? ctx->adj.as<unsigned long long>() : nullptr;
   L.mom = mom ? ctx->mom.as<double>() : nullptr;
-  L.queue = ctx->queue.as<unsigned long long>();
-  L.stats = ctx->stats.as<unsigned long long>();
+  L.queue = B.queue;
+  L.stats = B.stats;
   L.adjoint = adj ? 1 : 0;
   L.accum_fp64 = fp64 ? 1 : 0;
   L.kernel = kernel;
@@ -593,7 +649,8 @@ int trace(rti_ctx* ctx) {
   const bool adj = ctx->params.adjoint != 0;
   const bool fp64 = ctx->params.accum == RTI_ACCUM_FP64;
   const bool count = ctx->params.kernel == RTI_KERNEL_COUNT;
-  rc = upload_descriptors(ctx);
+  const trace_bufs B = ctx_bufs(ctx);
+  rc = upload_descriptors(ctx, B.desc);
   if (rc) return rc;
   rc = alloc_accumulators(ctx, adj);
   if (rc) return rc;
@@ -601,7 +658,7 @@ int trace(rti_ctx* ctx) {
   rc = pick_config(ctx, cfg, adj, fp64);
   if (rc) return rc;
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  rc = enqueue_trace(ctx, cfg, adj, fp64, count, ctx->params.kernel == RTI_KERNEL_PLAIN ? 1 : 0, -1);
+  rc = enqueue_trace(ctx, B, cfg, adj, fp64, count, ctx->params.kernel == RTI_KERNEL_PLAIN ? 1 : 0, -1);
   if (rc) return rc;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -614,11 +671,12 @@ int trace(rti_ctx* ctx) {
   ctx->last_moments = ctx->params.moments != 0;
   HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
   ctx->have_results = false;
+  ctx->acc_traced = true;
   return RTI_OK;
 }
 
 int finalize(rti_ctx* ctx) {
-  if (ctx->dirty) return fail(ctx, RTI_ERR_STATE, "no trace for the current geometry");
+  if (ctx->dirty || !ctx->acc_traced) return fail(ctx, RTI_ERR_STATE, "no trace for the current geometry");
   hipEvent_t a, b;
   HIP_TRY(ctx, hipEventCreate(&a));
   HIP_TRY(ctx, hipEventCreate(&b));
@@ -808,9 +866,15 @@ int rti_ctx_create(rti_ctx** out, int device) {
   rti_ctx* ctx = new rti_ctx;
   ctx->device = device;
   rti_default_params(&ctx->params);
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+  // the device address of the static scratch is resolved here, once per
+  // device, with the device current
+  if (hipSetDevice(device) != hipSuccess || resolve_scratch(device) != hipSuccess ||
+      hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+      hipEventCreate(&ctx->ev1) != hipSuccess ||
       hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RTI_ERR_HIP;
   }
@@ -1092,8 +1156,9 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times) {
   const bool adj = ctx->params.adjoint != 0;
   const bool fp64 = ctx->params.accum == RTI_ACCUM_FP64;
   const int kernel = ctx->params.kernel == RTI_KERNEL_PLAIN ? 1 : 0;
+  const trace_bufs B = ctx_bufs(ctx);
   if (!ctx->graph_exec) {
-    rc = upload_descriptors(ctx);
+    rc = upload_descriptors(ctx, B.desc);
     if (rc) return rc;
     rc = alloc_accumulators(ctx, adj);
     if (rc) return rc;
@@ -1102,7 +1167,7 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times) {
     if (rc) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_trace(ctx, cfg, adj, fp64, false, kernel, -1);
+    rc = enqueue_trace(ctx, B, cfg, adj, fp64, false, kernel, -1);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) return rc;
@@ -1123,6 +1188,56 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times) {
   ctx->last_moments = ctx->params.moments != 0;
   HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
   ctx->have_results = false;
+  if (times) ctx->acc_traced = true;
+  return RTI_OK;
+}
+
+int rti_trace_noalloc(rti_ctx* ctx, unsigned long long* acc, uint64_t acc_words) {
+  if (!ctx) return RTI_ERR_INVALID_ARG;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (ctx->inst.empty()) return fail(ctx, RTI_ERR_NO_GEOMETRY, "no geometry set");
+  if (ctx->dirty)
+    return fail(ctx, RTI_ERR_STATE, "the scene is not built for the current geometry: call rti_prepare first");
+  if (!acc) return fail(ctx, RTI_ERR_INVALID_ARG, "null accumulator buffer");
+  const uint64_t need = ctx->n_total * rti_dev::kAccStride;
+  if (acc_words < need)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "accumulator buffer holds %llu words, the scene needs %llu",
+                (unsigned long long)acc_words, (unsigned long long)need);
+  if (ctx->inst.size() > kNoAllocMaxInstances)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "%zu instances: the no-allocation trace takes at most %u", ctx->inst.size(),
+                kNoAllocMaxInstances);
+  if (ctx->params.adjoint || ctx->params.moments)
+    return fail(ctx, RTI_ERR_INVALID_ARG,
+                "adjoint and moments need accumulators the no-allocation trace does not take: use rti_trace");
+  unsigned char* base = static_cast<unsigned char*>(scratch_ptr(ctx->device));
+  if (!base) return fail(ctx, RTI_ERR_STATE, "static scratch not resolved for device %d", ctx->device);
+  const trace_bufs B{acc, reinterpret_cast<inst_desc*>(base + kScratchDescOffset),
+                     reinterpret_cast<unsigned long long*>(base),
+                     reinterpret_cast<unsigned long long*>(base + kScratchStatsOffset)};
+  const bool fp64 = ctx->params.accum == RTI_ACCUM_FP64;
+  const bool count = ctx->params.kernel == RTI_KERNEL_COUNT;
+  int rc = upload_descriptors(ctx, B.desc);
+  if (rc) return rc;
+  kernel_cfg cfg;
+  rc = pick_config(ctx, cfg, false, fp64);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  rc = enqueue_trace(ctx, B, cfg, false, fp64, count, ctx->params.kernel == RTI_KERNEL_PLAIN ? 1 : 0, -1);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->trace_ms = ms;
+  ctx->last_adjoint = 0;
+  ctx->last_fp64 = fp64;
+  ctx->last_count = count;
+  ctx->last_moments = 0;
+  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, B.stats, 64, hipMemcpyDeviceToHost));
+  // the context's getters and rti_finalize refer to its own accumulators,
+  // which this trace did not write
+  ctx->have_results = false;
+  ctx->acc_traced = false;
   return RTI_OK;
 }
 
