@@ -126,6 +126,13 @@ typedef struct rti_params {
   int32_t refill_lanes;   /* persistent kernel: a wave replaces ended rays once this many
                              of its lanes wait for one, or when none is alive (1..64);
                              0 = default (1, at every shading pass)                     */
+  uint64_t ray_begin;     /* id of the first ray: the run traces rays [ray_begin,
+                             ray_begin + n_rays) of the sequence a larger run with the
+                             same seed and instance would trace (a shard of a job)     */
+  uint64_t rays_total;    /* rays of the whole job this run is a shard of; sets the
+                             fixed-point scale, so that the integer sums of shards
+                             [0, a) + [a, b) + ... add up to the single run's bit for
+                             bit; 0 = n_rays; otherwise at least n_rays               */
 } rti_params;
 
 typedef struct rti_stats {
@@ -145,9 +152,16 @@ typedef struct rti_stats {
   float src_z;
   uint32_t wide_stack;    /* most stack entries a four-wide traversal can hold               */
   uint64_t wide_nodes;    /* four-wide traversal-tree nodes                                  */
+  int32_t fixed_shift2;   /* fixed-point scale of the sum of squared deposits is 2^fixed_shift2 */
+  int32_t pad0;
 } rti_stats;
 
-/* One geometry of a batch. Exactly one of (xyzr, nrm) or (verts, tris) is set. */
+/* One geometry of a batch. Exactly one of (xyzr, nrm) or (verts, tris) is set.
+ * Instances whose geometry buffers are the very same pointers and sizes as an
+ * earlier instance's (particle species on one geometry) share that instance's
+ * primitive records, trees and neighbour graph in device memory; each keeps
+ * its own parameters and accumulators, and results are the same as with
+ * separate copies. */
 typedef struct rti_instance {
   const float* xyzr;      /* n_prims x 4 */
   const float* nrm;       /* n_prims x 3, unit length */

@@ -116,6 +116,13 @@ def _declare(L):
     L.orc_source_origins.argtypes = [ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_uint64, _fp]
     L.orc_source_origins.restype = None
     L.orc_diffuse_dir.argtypes = [_fp, ctypes.c_float, ctypes.c_float, _fp]
+    L.orc_source_rays.argtypes = [ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_uint64, _fp, _fp]
+    L.orc_source_rays.restype = None
+    L.orc_specular_dir.argtypes = [_fp, _fp, _fp]
+    L.orc_specular_dir.restype = None
+    L.orc_closest_hits.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params), ctypes.c_uint64, _fp, _fp,
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _fp]
+    L.orc_closest_hits.restype = None
     L.orc_roulette.argtypes = [_fp, ctypes.c_float]
     L.orc_roulette.restype = ctypes.c_int
     L.orc_trace.restype = ctypes.c_int
@@ -168,6 +175,78 @@ def source_origins(box, src_z, n, source=0, seed=1, instance=0, ray_begin=0):
     out = np.zeros((int(n), 3), dtype=np.float32)
     lib().orc_source_origins(ctypes.byref(p), int(ray_begin), int(n), _ptr(out, _fp))
     return out
+
+
+def source_rays(box, src_z, n, cos_exponent=1.0, source=0, seed=1, instance=0, ray_begin=0):
+    """Origins and directions (each n x 3, float32) of the primary rays
+    [ray_begin, ray_begin + n) of a run with these parameters."""
+    p = Params()
+    p.seed, p.instance, p.source = int(seed), int(instance), int(source)
+    p.cos_exponent = float(cos_exponent)
+    for i in range(6):
+        p.box[i] = float(box[i])
+    p.src_z = float(src_z)
+    org = np.zeros((int(n), 3), dtype=np.float32)
+    dirn = np.zeros((int(n), 3), dtype=np.float32)
+    lib().orc_source_rays(ctypes.byref(p), int(ray_begin), int(n), _ptr(org, _fp), _ptr(dirn, _fp))
+    return org, dirn
+
+
+def specular_dir(d, nrm):
+    dd = np.ascontiguousarray(d, dtype=np.float32)
+    nn = np.ascontiguousarray(nrm, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_specular_dir(_ptr(dd, _fp), _ptr(nn, _fp), _ptr(out, _fp))
+    return out
+
+
+def _scene_struct(scene, tie_rank=None):
+    keep = []
+    sc = Scene()
+    if scene["kind"] == "discs":
+        xyzr = np.ascontiguousarray(scene["xyzr"], dtype=np.float32)
+        nrm = np.ascontiguousarray(scene["normals"], dtype=np.float32)
+        keep += [xyzr, nrm]
+        sc.prim_kind = 0
+        sc.xyzr = _ptr(xyzr, _fp)
+        sc.nrm = _ptr(nrm, _fp)
+        sc.n_prims = xyzr.shape[0]
+    else:
+        verts = np.ascontiguousarray(scene["verts"], dtype=np.float32)
+        tris = np.ascontiguousarray(scene["tris"], dtype=np.uint32)
+        keep += [verts, tris]
+        sc.prim_kind = 1
+        sc.verts = _ptr(verts, _fp)
+        sc.n_verts = verts.shape[0]
+        sc.tris = _ptr(tris, _u32p)
+        sc.n_prims = tris.shape[0]
+    if tie_rank is not None:
+        tr = np.ascontiguousarray(tie_rank, dtype=np.uint32)
+        keep.append(tr)
+        sc.tie_rank = _ptr(tr, _u32p)
+    return sc, keep
+
+
+def closest_hits(scene, box, org, dirn, tnear=1e-4, tie_rank=None):
+    """One closest-hit query per ray (the oracle's BVH plus the boundary
+    faces of ``box``): (prim, face, t) arrays; prim -1 when no primitive is
+    hit, face -1 unless the boundary is returned."""
+    sc, keep = _scene_struct(scene, tie_rank)
+    p = Params()
+    for i in range(6):
+        p.box[i] = float(box[i])
+    p.tnear = float(tnear)
+    o = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirn, dtype=np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    prim = np.zeros(n, dtype=np.int32)
+    face = np.zeros(n, dtype=np.int32)
+    t = np.zeros(n, dtype=np.float32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib().orc_closest_hits(ctypes.byref(sc), ctypes.byref(p), n, _ptr(o, _fp), _ptr(d, _fp), prim.ctypes.data_as(i32p),
+                           face.ctypes.data_as(i32p), _ptr(t, _fp))
+    del keep
+    return prim, face, t
 
 
 def source_dir(r1, r2, n):

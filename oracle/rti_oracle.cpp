@@ -898,6 +898,51 @@ void orc_source_dir(float r1, float r2, float n, float out[3]) {
   out[0] = d.x; out[1] = d.y; out[2] = d.z;
 }
 
+// Primary rays [r0, r0 + n): origins and directions exactly as trace_range
+// starts them (the rays a run with the same params traces).
+void orc_source_rays(const orc_params* p, uint64_t r0, uint64_t n, float* org, float* dir) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t ray = r0 + i;
+    const uint32_t clo = (uint32_t)ray, chi = (uint32_t)(ray >> 32);
+    const u32x4 u = philox({{clo, chi, 0u, 0u}}, p->seed, p->instance);
+    const v3<float> o = source_origin<float>(*p, u, clo, chi);
+    float lncos;
+    const v3<float> d = source_dir<float>(u01(u.v[2]), u01_open0(u.v[3]), p->cos_exponent, lncos);
+    org[3 * i] = o.x; org[3 * i + 1] = o.y; org[3 * i + 2] = o.z;
+    dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
+  }
+}
+
+// Mirror direction (specular::use, src/rti/reflection/specular.hpp:16-40),
+// exposed for the invariant and method-of-images tests.
+void orc_specular_dir(const float d[3], const float nrm[3], float out[3]) {
+  const v3<float> r = specular_dir<float>({d[0], d[1], d[2]}, {nrm[0], nrm[1], nrm[2]});
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+// The closest-hit query of one segment, as trace_range makes it, for n
+// rays: the boundary faces of p->box bound the primitive query (own BVH).
+// Per ray: prim = the closest primitive hit in [tnear, boundary distance]
+// (-1 none), face = the boundary face when no primitive is hit (-1 none),
+// t = the distance of whichever was returned (FLT_MAX for a miss).
+void orc_closest_hits(const orc_scene* sc, const orc_params* p, uint64_t n, const float* org, const float* dir,
+                      int32_t* prim, int32_t* face, float* t) {
+  scene_view<float> s;
+  make_scene<float>(sc, s);
+  for (uint64_t i = 0; i < n; ++i) {
+    const v3<float> o = {org[3 * i], org[3 * i + 1], org[3 * i + 2]};
+    const v3<float> d = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    float tb;
+    const int f = boundary_hit(p->box, o, d, p->tnear, tb);
+    float tp = tb;
+    uint32_t pb = 0;
+    const bool hit = closest_hit(s, o, d, p->tnear, tp, pb);
+    prim[i] = hit ? (int32_t)pb : -1;
+    face[i] = hit ? -1 : f;
+    t[i] = hit ? tp : tb;
+  }
+}
+
 void orc_diffuse_dir(const float nrm[3], float r1, float r2, float out[3]) {
   v3<float> d = diffuse_dir<float>({nrm[0], nrm[1], nrm[2]}, r1, r2);
   out[0] = d.x; out[1] = d.y; out[2] = d.z;

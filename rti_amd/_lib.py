@@ -59,6 +59,7 @@ class Params(ctypes.Structure):
         ("lds_stack", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("compact_lanes", ctypes.c_int32),
         ("bvh_width", ctypes.c_int32), ("moments", ctypes.c_int32), ("source", ctypes.c_int32),
         ("leaf_batch", ctypes.c_int32), ("tree_builder", ctypes.c_int32), ("refill_lanes", ctypes.c_int32),
+        ("ray_begin", ctypes.c_uint64), ("rays_total", ctypes.c_uint64),
     ]
 
 
@@ -69,7 +70,7 @@ class Stats(ctypes.Structure):
         ("n_edges", ctypes.c_uint64), ("bvh_depth", ctypes.c_uint32), ("fixed_shift", ctypes.c_int32),
         ("trace_ms", ctypes.c_double), ("build_ms", ctypes.c_double), ("post_ms", ctypes.c_double),
         ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float), ("wide_stack", ctypes.c_uint32),
-        ("wide_nodes", ctypes.c_uint64),
+        ("wide_nodes", ctypes.c_uint64), ("fixed_shift2", ctypes.c_int32), ("pad0", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -342,21 +343,26 @@ class Context:
         """items: list of dicts with keys scene, and rti_params fields."""
         arr = (Instance * len(items))()
         keep = []
+        converted = {}  # one buffer pair per scene object: items on the same scene share its device copy
         self.batch_n = []
         for k, it in enumerate(items):
             sc = it["scene"]
             kw = {key: val for key, val in it.items() if key not in ("scene",) and hasattr(Params, key)}
             arr[k].params = default_params(**kw)
             if sc["kind"] == "discs":
-                x = np.ascontiguousarray(sc["xyzr"], dtype=np.float32)
-                n = np.ascontiguousarray(sc["normals"], dtype=np.float32)
+                if id(sc) not in converted:
+                    converted[id(sc)] = (np.ascontiguousarray(sc["xyzr"], dtype=np.float32),
+                                         np.ascontiguousarray(sc["normals"], dtype=np.float32))
+                x, n = converted[id(sc)]
                 keep += [x, n]
                 arr[k].xyzr = _ptr(x, _fp)
                 arr[k].nrm = _ptr(n, _fp)
                 arr[k].n_prims = x.shape[0]
             else:
-                v = np.ascontiguousarray(sc["verts"], dtype=np.float32)
-                t = np.ascontiguousarray(sc["tris"], dtype=np.uint32)
+                if id(sc) not in converted:
+                    converted[id(sc)] = (np.ascontiguousarray(sc["verts"], dtype=np.float32),
+                                         np.ascontiguousarray(sc["tris"], dtype=np.uint32))
+                v, t = converted[id(sc)]
                 keep += [v, t]
                 arr[k].verts = _ptr(v, _fp)
                 arr[k].n_verts = v.shape[0]

@@ -67,8 +67,10 @@ struct instance {
   // geometry bounds (set with the geometry)
   float geo_lo[3] = {0, 0, 0}, geo_hi[3] = {0, 0, 0};
   float ztop = 0;
+  int share = -1;  // >= 0: this instance uses the geometry of instance `share` (same buffers)
   // derived at prepare
   uint32_t prim_begin = 0, node_begin = 0;
+  uint32_t acc_begin = 0;  // first accumulator slot
   uint64_t ray_begin = 0;
   float box[6] = {0, 0, 0, 0, 0, 0};
   float src_z = 0;
@@ -76,7 +78,7 @@ struct instance {
   uint32_t depth = 0;
   uint32_t n_wide = 0, wide_stack = 0;
   uint64_t n_edges = 0;
-  int shift = 32;
+  int shift = 32, shift2 = 32;  // fixed-point scales of the sum and of the sum of squares
   // results (input order)
   std::vector<double> raw, sigma, dds, ddn, rel, vov;
   std::vector<float> smoothed, areas;
@@ -106,6 +108,7 @@ struct rti_ctx {
   dbuf raw_st, vtmp, maxbuf, out_raw, out_sigma, out_cnt, out_smooth, out_f, out_adj, out_err;
   rti_impl::build_scratch scratch;
   uint64_t n_total = 0, nodes_total = 0, edges_total = 0, rays_total = 0;
+  uint64_t acc_total = 0;  // accumulator slots: the primitives of every instance
   double build_ms = 0, trace_ms = 0, post_ms = 0;
   int last_adjoint = 0, last_fp64 = 0, last_count = 0, last_moments = 0;
   uint64_t run_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -172,6 +175,12 @@ int fail(rti_ctx* ctx, int code, const char* fmt, ...) {
                   #expr, hipGetErrorString(e_), __FILE__, __LINE__);                                  \
   } while (0)
 
+// Deepest binary tree the binary traversal and the neighbour search accept
+// (their stacks hold 64 entries). Karras trees over 30-bit Morton codes stay
+// below it; PLOC trees are not bounded, and the four-wide traversal checks its
+// own exact stack bound instead.
+constexpr int kMaxBinaryDepth = 64;
+
 int check_params(rti_ctx* ctx, const rti_params& p) {
   if (!(p.sticking > 0.0f && p.sticking <= 1.0f))
     return fail(ctx, RTI_ERR_INVALID_ARG, "sticking must be in (0, 1], got %g", (double)p.sticking);
@@ -201,6 +210,10 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
   if (p.moments != 0 && p.moments != 1) return fail(ctx, RTI_ERR_INVALID_ARG, "moments must be 0 or 1");
   if (p.source != RTI_SOURCE_RECTANGLE && p.source != RTI_SOURCE_DISC)
     return fail(ctx, RTI_ERR_INVALID_ARG, "source must be RTI_SOURCE_RECTANGLE or RTI_SOURCE_DISC");
+  if (p.rays_total != 0 && p.rays_total < p.n_rays)
+    return fail(ctx, RTI_ERR_INVALID_ARG, "rays_total (%llu) must be 0 or at least n_rays (%llu)",
+                (unsigned long long)p.rays_total, (unsigned long long)p.n_rays);
+  if (p.ray_begin + p.n_rays < p.ray_begin) return fail(ctx, RTI_ERR_INVALID_ARG, "ray range wraps around");
   return RTI_OK;
 }
 
@@ -357,17 +370,29 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, hipEventCreate(&b1));
   HIP_TRY(ctx, hipEventRecord(b0, ctx->stream));
   // layout
-  uint64_t np = 0, nn = 0, maxn = 0;
+  // an instance that shares another's geometry takes that instance's
+  // primitive records, trees and neighbour rows; every instance has its own
+  // accumulator slots
+  uint64_t np = 0, nn = 0, maxn = 0, na = 0;
   for (auto& I : ctx->inst) {
-    I.prim_begin = (uint32_t)np;
-    I.node_begin = (uint32_t)nn;
-    np += I.n;
-    nn += std::max<uint64_t>(I.n - 1, 1);
-    maxn = std::max(maxn, I.n);
+    if (I.share < 0) {
+      I.prim_begin = (uint32_t)np;
+      I.node_begin = (uint32_t)nn;
+      np += I.n;
+      nn += std::max<uint64_t>(I.n - 1, 1);
+      maxn = std::max(maxn, I.n);
+    } else {
+      I.prim_begin = ctx->inst[I.share].prim_begin;
+      I.node_begin = ctx->inst[I.share].node_begin;
+    }
+    I.acc_begin = (uint32_t)std::min<uint64_t>(na, UINT32_MAX);
+    na += I.n;
     derive_box(I);
   }
   if (np >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives in total");
+  if (na >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives over all instances");
   ctx->n_total = np;
+  ctx->acc_total = na;
   ctx->nodes_total = nn;
   // 32-byte records when every primitive is a disc, else 64-byte records
   bool discs_only = true;
@@ -380,7 +405,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
   HIP_TRY(ctx, ctx->nb_ptr.ensure((np + 1) * 4));
   HIP_TRY(ctx, ctx->areas.ensure(np * 4));
-  HIP_TRY(ctx, ctx->acc.ensure(np * 8 * rti_dev::kAccStride));
+  HIP_TRY(ctx, ctx->acc.ensure(na * 8 * rti_dev::kAccStride));
   HIP_TRY(ctx, ctx->queue.ensure(64));
   HIP_TRY(ctx, ctx->stats.ensure(64));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
@@ -394,6 +419,7 @@ int prepare(rti_ctx* ctx) {
     instance& I = ctx->inst[k];
     rti_impl::build_job& J = jobs[k];
     J = rti_impl::build_job{};
+    if (I.share >= 0) continue;  // built with the instance it shares
     J.prim_kind = I.kind;
     J.n = (uint32_t)I.n;
     J.prim_begin = I.prim_begin;
@@ -423,7 +449,11 @@ int prepare(rti_ctx* ctx) {
     uint32_t depth = 0;
     HIP_TRY(ctx, rti_impl::build_instance(J, st, ctx->scratch, &depth, ctx->stream));
     I.depth = depth;
-    if (depth > 64) return fail(ctx, RTI_ERR_HIP, "hierarchy depth %u exceeds the traversal bound 64", depth);
+    // the neighbour search walks the binary tree with a 64-entry stack
+    if (depth > kMaxBinaryDepth && I.kind == 0)
+      return fail(ctx, RTI_ERR_INVALID_ARG,
+                  "hierarchy depth %u exceeds the neighbour search's bound %d: use tree_builder = RTI_TREE_LBVH",
+                  depth, kMaxBinaryDepth);
     HIP_TRY(ctx, rti_impl::build_wide(J, st, ctx->scratch, &I.n_wide, &I.wide_stack, ctx->stream));
     // neighbour degrees -> row offsets (absolute)
     uint32_t* rowp = ctx->nb_ptr.as<uint32_t>() + I.prim_begin;
@@ -448,6 +478,14 @@ int prepare(rti_ctx* ctx) {
     HIP_TRY(ctx, hipGetLastError());
   }
   ctx->edges_total = edges;
+  for (auto& I : ctx->inst) {
+    if (I.share < 0) continue;
+    const instance& O = ctx->inst[I.share];
+    I.depth = O.depth;
+    I.n_wide = O.n_wide;
+    I.wide_stack = O.wide_stack;
+    I.n_edges = O.n_edges;
+  }
   k_link_rows<<<(unsigned)((np + 1 + 255) / 256), 256, 0, ctx->stream>>>(ctx->prims.as<float4>(), ctx->prim_shift,
                                                                          ctx->nb_ptr.as<uint32_t>(), (uint32_t)np);
   HIP_TRY(ctx, hipGetLastError());
@@ -455,7 +493,7 @@ int prepare(rti_ctx* ctx) {
   st.nb_col = ctx->nb_col.as<uint32_t>();
   for (size_t k = 0; k < ctx->inst.size(); ++k) {
     instance& I = ctx->inst[k];
-    if (I.kind != 0 || I.n_edges == 0) continue;
+    if (I.kind != 0 || I.n_edges == 0 || I.share >= 0) continue;
     // inputs are not needed: the fill pass reads the stored prims
     HIP_TRY(ctx, rti_impl::neighbor_fill(jobs[k], st, ctx->stream));
   }
@@ -521,16 +559,25 @@ int upload_descriptors(rti_ctx* ctx, inst_desc* dst) {
     d.root4 = (int32_t)I.node_begin;
     d.source = p.source;
     d.ray_begin = rb;
-    d.n_rays = p.n_rays;
+    d.ray_id0 = p.ray_begin;
     d.seed = p.seed;
     d.key1 = p.instance + (uint32_t)k;
     d.roulette = p.roulette ? 1u : 0u;
-    d.prim_begin = I.prim_begin;
+    d.acc_off = (int32_t)I.acc_begin - (int32_t)I.prim_begin;
     d.n_prims = (uint32_t)I.n;
-    // 2^shift with shift = 58 - bitlen(n_rays), at most 40: an element can
-    // only overflow after more than 32 deposits per ray on average
-    I.shift = std::min(40, 58 - bit_length(p.n_rays));
+    // 2^shift with shift = 58 - bitlen(rays of the job), at most 40: each
+    // deposit adds at most 2^shift, so an element can only overflow after
+    // more than 32 deposits per ray on average (rti_finalize checks)
+    I.shift = std::min(40, 58 - bit_length(p.rays_total ? p.rays_total : p.n_rays));
     d.fx_scale = std::ldexp(1.0f, I.shift);
+    // sum of squares: 2^shift2 with s^2 2^shift2 <= 2^(shift - 1) for s < 1
+    // (2^shift for s = 1), so a squared deposit (<= s^2, rounded in FP32) adds
+    // no more than a deposit can (the same overflow bound), and the smallest
+    // squared deposit with rejection control, (0.1 s)^2, still spans more
+    // than 2^(shift - 9) grid units
+    const int lg = (int)std::floor(std::log2(1.0 / ((double)p.sticking * (double)p.sticking)));
+    I.shift2 = I.shift + std::max(0, lg - 1);
+    d.fx_scale2 = std::ldexp(1.0f, I.shift2);
     I.ray_begin = rb;
     rb += p.n_rays;
   }
@@ -548,6 +595,15 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
   const rti_params& p = ctx->params;
   cfg.block = p.block_size ? p.block_size : 256;
   cfg.width = p.bvh_width ? p.bvh_width : 4;
+  if (cfg.width == 2) {
+    // the binary traversal's lane stack holds 64 entries
+    for (auto& I : ctx->inst)
+      if ((int)I.depth > kMaxBinaryDepth)
+        return fail(ctx, RTI_ERR_INVALID_ARG,
+                    "binary traversal needs a hierarchy of depth <= %d, this one has %u: use bvh_width = 4 or "
+                    "tree_builder = RTI_TREE_LBVH",
+                    kMaxBinaryDepth, I.depth);
+  }
   cfg.leaf = p.leaf_batch < 0 ? 0
              : (p.leaf_batch ? p.leaf_batch : rti_impl::default_leaf_batch(ctx->prim_shift == rti_dev::kPrimShiftMixed));
   cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width, cfg.width == 4 && cfg.leaf > 0);
@@ -567,8 +623,8 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
 }
 
 int alloc_accumulators(rti_ctx* ctx, bool adj) {
-  if (adj) HIP_TRY(ctx, ctx->adj.ensure(ctx->n_total * 16));
-  if (ctx->params.moments) HIP_TRY(ctx, ctx->mom.ensure(ctx->n_total * 16));
+  if (adj) HIP_TRY(ctx, ctx->adj.ensure(ctx->acc_total * 16));
+  if (ctx->params.moments) HIP_TRY(ctx, ctx->mom.ensure(ctx->acc_total * 16));
   return RTI_OK;
 }
 
@@ -592,7 +648,7 @@ trace_bufs ctx_bufs(rti_ctx* ctx) {
 // `only` selects one instance for the plain per-geometry path (-1: all).
 int enqueue_trace(rti_ctx* ctx, const trace_bufs& B, const kernel_cfg& cfg, bool adj, bool fp64, bool count,
                   int kernel, long only) {
-  const uint64_t np = ctx->n_total;
+  const uint64_t np = ctx->acc_total;  // accumulator slots
   HIP_TRY(ctx, rti_impl::fill_u64(B.acc, np * rti_dev::kAccStride, 0ull, ctx->stream));
   if (adj) HIP_TRY(ctx, rti_impl::fill_u64(ctx->adj.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
   const bool mom = ctx->params.moments != 0;
@@ -695,10 +751,10 @@ int finalize(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->out_err.ensure(maxn * 16));
   for (auto& I : ctx->inst) {
     const uint32_t n = (uint32_t)I.n;
-    const double inv = std::ldexp(1.0, -I.shift);
+    const double inv = std::ldexp(1.0, -I.shift), inv2 = std::ldexp(1.0, -I.shift2);
     HIP_TRY(ctx, rti_impl::finalize_instance(
-                     ctx->acc.as<unsigned long long>(), ctx->perm.as<uint32_t>(), I.prim_begin, n, inv,
-                     ctx->last_fp64, I.params.n_rays, ctx->raw_st.as<double>(), ctx->out_raw.as<double>(),
+                     ctx->acc.as<unsigned long long>(), I.acc_begin, ctx->perm.as<uint32_t>(), I.prim_begin, n, inv,
+                     inv2, ctx->last_fp64, I.params.n_rays, ctx->raw_st.as<double>(), ctx->out_raw.as<double>(),
                      ctx->out_sigma.as<double>(), ctx->out_cnt.as<unsigned long long>(), ctx->stream));
     HIP_TRY(ctx, rti_impl::smooth_normalize(ctx->raw_st.as<double>(), ctx->areas.as<float>(),
                                             ctx->perm.as<uint32_t>(), I.kind == 0 ? ctx->nb_ptr.as<uint32_t>() : nullptr,
@@ -720,10 +776,10 @@ int finalize(rti_ctx* ctx) {
       const double inva = std::ldexp(1.0, -(I.shift - 8));
       double* o1 = ctx->out_adj.as<double>();
       double* o2 = o1 + maxn;
-      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 0, ctx->perm.as<uint32_t>(),
-                                             I.prim_begin, n, inva, o1, ctx->stream));
-      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 1, ctx->perm.as<uint32_t>(),
-                                             I.prim_begin, n, inva, o2, ctx->stream));
+      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 0, I.acc_begin,
+                                             ctx->perm.as<uint32_t>(), I.prim_begin, n, inva, o1, ctx->stream));
+      HIP_TRY(ctx, rti_impl::finalize_signed(ctx->adj.as<unsigned long long>(), 1, I.acc_begin,
+                                             ctx->perm.as<uint32_t>(), I.prim_begin, n, inva, o2, ctx->stream));
       I.dds.resize(n);
       I.ddn.resize(n);
       HIP_TRY(ctx, hipMemcpyAsync(I.dds.data(), o1, n * 8ull, hipMemcpyDeviceToHost, ctx->stream));
@@ -733,8 +789,8 @@ int finalize(rti_ctx* ctx) {
       I.ddn.clear();
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    // overflow guard: each deposit adds at most 2^shift, so no element can
-    // have wrapped unless its deposit count reaches 2^(63 - shift)
+    // overflow guard: each deposit adds at most 2^shift to either sum, so no
+    // element can have wrapped unless its deposit count reaches 2^(63 - shift)
     if (!ctx->last_fp64) {
       uint64_t mc = 0;
       for (uint64_t c : I.counts) mc = std::max(mc, c);
@@ -750,8 +806,8 @@ int finalize(rti_ctx* ctx) {
     double* ev = er + maxn;
     const bool mom = ctx->last_moments != 0;
     HIP_TRY(ctx, rti_impl::error_stats(ctx->acc.as<unsigned long long>(), mom ? ctx->mom.as<double>() : nullptr,
-                                       ctx->perm.as<uint32_t>(), I.prim_begin, n, inv, ctx->last_fp64, total, er,
-                                       mom ? ev : nullptr, ctx->stream));
+                                       I.acc_begin, ctx->perm.as<uint32_t>(), I.prim_begin, n, inv, inv2,
+                                       ctx->last_fp64, total, er, mom ? ev : nullptr, ctx->stream));
     I.rel.resize(n);
     HIP_TRY(ctx, hipMemcpyAsync(I.rel.data(), er, n * 8ull, hipMemcpyDeviceToHost, ctx->stream));
     if (mom) {
@@ -801,9 +857,10 @@ void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   for (auto& J : ctx->inst) {
     np += J.n;
     nr += J.params.n_rays;
-    ne += J.n_edges;
     depth = std::max(depth, J.depth);
     wstack = std::max(wstack, J.wide_stack);
+    if (J.share >= 0) continue;  // its geometry is counted with the instance it shares
+    ne += J.n_edges;
     nw += J.n_wide;
   }
   out->n_prims = I ? I->n : np;
@@ -823,6 +880,7 @@ void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   out->src_z = J.src_z;
   out->wide_stack = I ? I->wide_stack : wstack;
   out->wide_nodes = I ? I->n_wide : nw;
+  out->fixed_shift2 = J.shift2;
 }
 
 int get_error_impl(rti_ctx* ctx, const instance& I, double* rel_err, double* vov) {
@@ -1114,6 +1172,30 @@ int rti_batch_set(rti_ctx* ctx, const rti_instance* in, uint32_t n) {
     int rc = check_params(ctx, s.params);
     if (rc) return rc;
     instance& I = v[k];
+    // the very same geometry buffers as an earlier instance: share its
+    // device copy (particle species on one geometry)
+    int owner = -1;
+    for (uint32_t j = 0; j < k && owner < 0; ++j) {
+      const rti_instance& o = in[j];
+      if (v[j].share >= 0 || o.n_prims != s.n_prims) continue;
+      const bool same = s.xyzr ? (o.xyzr == s.xyzr && o.nrm == s.nrm)
+                               : (!o.xyzr && o.verts == s.verts && o.tris == s.tris && o.n_verts == s.n_verts);
+      if (same) owner = (int)j;
+    }
+    if (owner >= 0) {
+      const instance& O = v[owner];
+      I.kind = O.kind;
+      I.n = O.n;
+      I.share = owner;
+      I.params = s.params;
+      I.rmax = O.rmax;
+      I.ztop = O.ztop;
+      for (int a = 0; a < 3; ++a) {
+        I.geo_lo[a] = O.geo_lo[a];
+        I.geo_hi[a] = O.geo_hi[a];
+      }
+      continue;
+    }
     if (s.xyzr) {
       rc = check_discs(ctx, s.xyzr, s.nrm, s.n_prims);
       if (rc) return rc;
@@ -1199,7 +1281,7 @@ int rti_trace_noalloc(rti_ctx* ctx, unsigned long long* acc, uint64_t acc_words)
   if (ctx->dirty)
     return fail(ctx, RTI_ERR_STATE, "the scene is not built for the current geometry: call rti_prepare first");
   if (!acc) return fail(ctx, RTI_ERR_INVALID_ARG, "null accumulator buffer");
-  const uint64_t need = ctx->n_total * rti_dev::kAccStride;
+  const uint64_t need = ctx->acc_total * rti_dev::kAccStride;
   if (acc_words < need)
     return fail(ctx, RTI_ERR_INVALID_ARG, "accumulator buffer holds %llu words, the scene needs %llu",
                 (unsigned long long)acc_words, (unsigned long long)need);

@@ -89,9 +89,10 @@ struct trace_launch {
   const rti_dev::bvh_node* nodes;
   const rti_dev::bvh4_node* nodes4;
   const uint32_t* nb_col;           // neighbour columns; row bounds are in the prims records
-  unsigned long long* acc;          // kAccStride per primitive: sum (fixed point or double bits), sum of squares, count
-  unsigned long long* adj;          // 2 per primitive: d/ds, d/dn (signed fixed point); may be null
-  double* mom;                      // 2 per primitive: sums of deposit^3, ^4 (FP64 atomics); may be null
+  unsigned long long* acc;          // kAccStride per accumulator slot: sum, sum of squares (fixed point or double
+                                    // bits), count
+  unsigned long long* adj;          // 2 per slot: d/ds, d/dn (signed fixed point); may be null
+  double* mom;                      // 2 per slot: sums of deposit^3, ^4 (FP64 atomics); may be null
   unsigned long long* queue;        // [0]: next ray id
   unsigned long long* stats;        // segments, surface hits, misses, truncated
   double fx_scale;                  // 2^shift
@@ -119,21 +120,25 @@ int default_leaf_batch(bool triangles);  // 0 = leaf tests at each node
 
 // ---- post ----
 hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);
-// Converts one instance's accumulators: raw_stored[j] (local stored order)
-// and, in input order via perm, raw_out / sigma_out / cnt_out (each may be null).
-hipError_t finalize_instance(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                             double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
-                             double* sigma_out, unsigned long long* cnt_out, hipStream_t stream);
+// Converts one instance's accumulators (slots acc_begin ..; primitives at
+// stored positions prim_begin ..): raw_stored[j] (local stored order) and, in
+// input order via perm, raw_out / sigma_out / cnt_out (each may be null).
+// inv_scale / inv_scale2: inverse fixed-point scales of the sum and the sum
+// of squares.
+hipError_t finalize_instance(const unsigned long long* acc, uint32_t acc_begin, const uint32_t* perm,
+                             uint32_t prim_begin, uint32_t n, double inv_scale, double inv_scale2, int accum_fp64,
+                             uint64_t n_rays, double* raw_stored, double* raw_out, double* sigma_out,
+                             unsigned long long* cnt_out, hipStream_t stream);
 // Relative error (rel_out) and, when s3 / s4 are given, the variance of the
 // variance (vov_out) per primitive in input order; total = all deposits of
 // the instance. Either output may be null.
-hipError_t error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t prim_begin,
-                       uint32_t n, double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
-                       hipStream_t stream);
-// Signed fixed-point (adjoint) accumulator component k of adj[2p + k] to
+hipError_t error_stats(const unsigned long long* acc, const double* mom, uint32_t acc_begin, const uint32_t* perm,
+                       uint32_t prim_begin, uint32_t n, double inv_scale, double inv_scale2, int accum_fp64,
+                       uint64_t total, double* rel_out, double* vov_out, hipStream_t stream);
+// Signed fixed-point (adjoint) accumulator component k of adj[2a + k] to
 // double, input order.
-hipError_t finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                           double inv_scale, double* out, hipStream_t stream);
+hipError_t finalize_signed(const unsigned long long* adj, int k, uint32_t acc_begin, const uint32_t* perm,
+                           uint32_t prim_begin, uint32_t n, double inv_scale, double* out, hipStream_t stream);
 // Smoothed, max-normalised estimate (reference device.hpp:153-187), written
 // in input order. raw_stored / areas are in local stored order; the CSR uses
 // absolute ids. vtmp: n doubles; maxbuf: 1 double.

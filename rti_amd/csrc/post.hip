@@ -27,13 +27,15 @@ __device__ __forceinline__ double acc_value(unsigned long long x, double inv_sca
   return fp64 ? __longlong_as_double((long long)x) : (double)(long long)x * inv_scale;
 }
 
-__global__ void k_finalize(const unsigned long long* acc, const uint32_t* perm, uint32_t pb, uint32_t n,
-                           double inv_scale, int fp64, double nrays, double* raw_stored, double* raw_out,
-                           double* sigma_out, unsigned long long* cnt_out) {
+// Primitive j of an instance: stored position pb + j (perm, areas), accumulator
+// slot ab + j (acc, adj, mom).
+__global__ void k_finalize(const unsigned long long* acc, uint32_t ab, const uint32_t* perm, uint32_t pb, uint32_t n,
+                           double inv_scale, double inv_scale2, int fp64, double nrays, double* raw_stored,
+                           double* raw_out, double* sigma_out, unsigned long long* cnt_out) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   size_t g = (size_t)pb + j;
-  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * g;
+  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * ((size_t)ab + j);
   double v = acc_value(a[0], inv_scale, fp64);
   raw_stored[j] = v;
   uint32_t i = perm[g];
@@ -41,7 +43,7 @@ __global__ void k_finalize(const unsigned long long* acc, const uint32_t* perm, 
   if (sigma_out) {
     // standard deviation of the sum over nrays histories, from the
     // per-deposit second moment: var = S2 - S1^2 / N
-    double q = acc_value(a[1], inv_scale, fp64);
+    double q = acc_value(a[1], inv_scale2, fp64);
     double var = q - v * v / nrays;
     sigma_out[i] = var > 0.0 ? sqrt(var) : 0.0;
   }
@@ -57,16 +59,18 @@ __global__ void k_finalize(const unsigned long long* acc, const uint32_t* perm, 
 //                                                      (DBL_MAX where the count or
 //                                                       the denominator is 0)
 // The powers of N are formed in double.
-__global__ void k_error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t pb,
-                              uint32_t n, double inv_scale, int fp64, double nn, double* rel, double* vov) {
+__global__ void k_error_stats(const unsigned long long* acc, const double* mom, uint32_t ab, const uint32_t* perm,
+                              uint32_t pb, uint32_t n, double inv_scale, double inv_scale2, int fp64, double nn,
+                              double* rel, double* vov) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const size_t g = (size_t)pb + j;
+  const size_t sl = (size_t)ab + j;
   const uint32_t i = perm[g];
   const double big = DBL_MAX;
-  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * g;
+  const unsigned long long* a = acc + (size_t)rti_dev::kAccStride * sl;
   const double a1 = acc_value(a[0], inv_scale, fp64);
-  const double a2 = acc_value(a[1], inv_scale, fp64);
+  const double a2 = acc_value(a[1], inv_scale2, fp64);
   const double sq = a1 * a1;
   if (rel) rel[i] = (nn == 0.0 || sq == 0.0) ? big : sqrt(a2 / sq - 1.0 / nn);
   if (vov) {
@@ -74,7 +78,7 @@ __global__ void k_error_stats(const unsigned long long* acc, const double* mom, 
       vov[i] = big;
       return;
     }
-    const double a3 = mom[2 * g], a4 = mom[2 * g + 1];
+    const double a3 = mom[2 * sl], a4 = mom[2 * sl + 1];
     const double q4 = sq * sq, a2sq = a2 * a2;
     const double num = a4 - (4 * a1 * a3 / nn) + (8 * a2 * sq / (nn * nn)) - (4 * q4 / (nn * nn * nn)) - (a2sq / nn);
     const double r2 = nn * a2 - sq;
@@ -83,12 +87,12 @@ __global__ void k_error_stats(const unsigned long long* acc, const double* mom, 
   }
 }
 
-__global__ void k_finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t pb, uint32_t n,
-                                  double inv_scale, double* out) {
+__global__ void k_finalize_signed(const unsigned long long* adj, int k, uint32_t ab, const uint32_t* perm, uint32_t pb,
+                                  uint32_t n, double inv_scale, double* out) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   size_t g = (size_t)pb + j;
-  out[perm[g]] = (double)(long long)adj[2 * g + k] * inv_scale;
+  out[perm[g]] = (double)(long long)adj[2 * ((size_t)ab + j) + k] * inv_scale;
 }
 
 __global__ void k_smooth(const double* raw, const float* areas, const uint32_t* nb_ptr, const uint32_t* nb_col,
@@ -138,26 +142,27 @@ hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hip
   return hipGetLastError();
 }
 
-hipError_t finalize_instance(const unsigned long long* acc, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                             double inv_scale, int accum_fp64, uint64_t n_rays, double* raw_stored, double* raw_out,
-                             double* sigma_out, unsigned long long* cnt_out, hipStream_t stream) {
-  k_finalize<<<nblk(n, 256), 256, 0, stream>>>(acc, perm, prim_begin, n, inv_scale, accum_fp64,
+hipError_t finalize_instance(const unsigned long long* acc, uint32_t acc_begin, const uint32_t* perm,
+                             uint32_t prim_begin, uint32_t n, double inv_scale, double inv_scale2, int accum_fp64,
+                             uint64_t n_rays, double* raw_stored, double* raw_out, double* sigma_out,
+                             unsigned long long* cnt_out, hipStream_t stream) {
+  k_finalize<<<nblk(n, 256), 256, 0, stream>>>(acc, acc_begin, perm, prim_begin, n, inv_scale, inv_scale2, accum_fp64,
                                                (double)(n_rays ? n_rays : 1), raw_stored, raw_out, sigma_out,
                                                cnt_out);
   return hipGetLastError();
 }
 
-hipError_t error_stats(const unsigned long long* acc, const double* mom, const uint32_t* perm, uint32_t prim_begin,
-                       uint32_t n, double inv_scale, int accum_fp64, uint64_t total, double* rel_out, double* vov_out,
-                       hipStream_t stream) {
-  k_error_stats<<<nblk(n, 256), 256, 0, stream>>>(acc, mom, perm, prim_begin, n, inv_scale, accum_fp64, (double)total,
-                                                  rel_out, vov_out);
+hipError_t error_stats(const unsigned long long* acc, const double* mom, uint32_t acc_begin, const uint32_t* perm,
+                       uint32_t prim_begin, uint32_t n, double inv_scale, double inv_scale2, int accum_fp64,
+                       uint64_t total, double* rel_out, double* vov_out, hipStream_t stream) {
+  k_error_stats<<<nblk(n, 256), 256, 0, stream>>>(acc, mom, acc_begin, perm, prim_begin, n, inv_scale, inv_scale2,
+                                                  accum_fp64, (double)total, rel_out, vov_out);
   return hipGetLastError();
 }
 
-hipError_t finalize_signed(const unsigned long long* adj, int k, const uint32_t* perm, uint32_t prim_begin, uint32_t n,
-                           double inv_scale, double* out, hipStream_t stream) {
-  k_finalize_signed<<<nblk(n, 256), 256, 0, stream>>>(adj, k, perm, prim_begin, n, inv_scale, out);
+hipError_t finalize_signed(const unsigned long long* adj, int k, uint32_t acc_begin, const uint32_t* perm,
+                           uint32_t prim_begin, uint32_t n, double inv_scale, double* out, hipStream_t stream) {
+  k_finalize_signed<<<nblk(n, 256), 256, 0, stream>>>(adj, k, acc_begin, perm, prim_begin, n, inv_scale, out);
   return hipGetLastError();
 }
 

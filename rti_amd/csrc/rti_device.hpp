@@ -21,11 +21,15 @@
 //   nb_ptr/nb_col  neighbour CSR over absolute prim ids (discs; the trace
 //                  reads the row bounds from the records, nb_ptr serves the
 //                  build, the smoothing and the exports)
-//   acc[4p + k]    u64  k = 0 sum of deposits, 1 sum of squares (fixed point,
-//                  or FP64 bits), 2 deposit count, 3 unused: one 32-byte
-//                  record, so a deposit's three atomics share a cache line
-//   adj[2p + k]    u64  adjoint sums d/ds (k = 0) and d/dn (k = 1), signed fixed point
-//   mom[2p + k]    f64  sums of deposit^3 (k = 0) and deposit^4 (k = 1)
+//   acc[4a + k]    u64  k = 0 sum of deposits (fixed point 2^shift), 1 sum of
+//                  squares (fixed point 2^shift2), or both as FP64 bits, 2
+//                  deposit count, 3 unused: one 32-byte record, so a
+//                  deposit's three atomics share a cache line. a is the
+//                  accumulator slot, p + acc_off of the primitive's instance:
+//                  instances (particle species) that share one geometry share
+//                  its primitive records and trees but not their slots
+//   adj[2a + k]    u64  adjoint sums d/ds (k = 0) and d/dn (k = 1), signed fixed point
+//   mom[2a + k]    f64  sums of deposit^3 (k = 0) and deposit^4 (k = 1)
 // Primitives are stored in Morton order of their instance, so neighbouring
 // primitives in space are neighbours in memory (coalesced leaf and
 // accumulator traffic); `perm` maps a stored position back to the input id.
@@ -84,9 +88,14 @@ struct __align__(16) inst_desc {
   float zmin, zmax, src_z, tnear;   // boundary z extent, source plane, ray epsilon
   float sticking, cos_exponent, diffuse_prob, pow_e;  // pow_e = 2 / (n + 1)
   int32_t bc_x, bc_y, prim_kind, root;                // root: absolute node index
-  uint64_t ray_begin, n_rays;       // slice of the global ray queue
-  uint32_t seed, key1, roulette, prim_begin;
-  uint32_t n_prims, pad0, pad1, pad2;
+  uint64_t ray_begin;               // first id of the instance's slice of the global ray queue
+  uint64_t ray_id0;                 // Philox ray id of the slice's first ray (the run's first ray id)
+  uint32_t seed, key1, roulette;
+  int32_t acc_off;                  // accumulator slot of primitive p: p + acc_off (species
+                                    // sharing one geometry have their own slots)
+  uint32_t n_prims;
+  float fx_scale2;                  // fixed-point scale of the sum of squares 2^shift2
+  uint32_t pad1, pad2;
   float score_c, fx_scale;          // score_c = 1 / (n + 1); fixed-point scale 2^shift
   int32_t root4, source;            // root4: absolute four-wide node index; source: rti_source
 };

@@ -51,9 +51,9 @@ struct tparams {
   const bvh_node* __restrict__ nodes;
   const bvh4_node* __restrict__ nodes4;
   const uint32_t* __restrict__ nb_col;
-  unsigned long long* acc;  // kAccStride per primitive: sum, sum of squares, count
-  unsigned long long* adj;  // 2 per primitive: d/ds, d/dn (adjoint only)
-  double* mom;              // 2 per primitive: sums of deposit^3, ^4 (or null)
+  unsigned long long* acc;  // kAccStride per accumulator slot: sum, sum of squares, count
+  unsigned long long* adj;  // 2 per slot: d/ds, d/dn (adjoint only)
+  double* mom;              // 2 per slot: sums of deposit^3, ^4 (or null)
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
@@ -506,33 +506,41 @@ __device__ __forceinline__ f3 diffuse_dir(f3 n, float r1, float r2) {
   return normalize(n * c1 + b1 * (c * st) + b2 * (s * st));
 }
 
+// One deposit v = drop on accumulator slot a. Fixed point: the products
+// v * 2^shift and the adjoint weights * 2^(shift - 8) are exact in FP32
+// (power-of-two scales) and are rounded once, to nearest, to 64-bit
+// integers; v^2 is formed in FP32 (one rounding, 2^-24 relative) and put on
+// its own grid 2^shift2, chosen from the sticking so that the smallest
+// squared deposit still spans many grid units. No FP64 arithmetic unless
+// FP64 accumulation or the higher moments are asked for.
 template <bool ADJ, bool FP64ACC, bool COUNT>
-__device__ __forceinline__ void deposit(const tparams& P, uint32_t p, float drop, float scale, float dds, float ddn,
-                                        counters& C) {
+__device__ __forceinline__ void deposit(const tparams& P, uint32_t a_slot, float drop, float scale, float scale2,
+                                        float dds, float ddn, counters& C) {
   if (COUNT) C.deps++;
-  const double v = (double)drop;
-  unsigned long long* a = P.acc + (size_t)kAccStride * p;  // one 32-byte record
+  unsigned long long* a = P.acc + (size_t)kAccStride * a_slot;  // one 32-byte record
   if (FP64ACC) {
+    const double v = (double)drop;
     unsafeAtomicAdd(reinterpret_cast<double*>(a), v);
     unsafeAtomicAdd(reinterpret_cast<double*>(a + 1), v * v);
   } else {
-    atomicAdd(a, (unsigned long long)__double2ll_rn(v * (double)scale));
-    atomicAdd(a + 1, (unsigned long long)__double2ll_rn(v * v * (double)scale));
+    atomicAdd(a, (unsigned long long)__float2ll_rn(drop * scale));
+    atomicAdd(a + 1, (unsigned long long)__float2ll_rn((drop * drop) * scale2));
   }
   atomicAdd(a + 2, 1ull);
   if (P.mom) {
-    // higher moments span too many orders of magnitude for the fixed-point
-    // scale of the first two; they use FP64 atomics (hit_accumulator.hpp:128-138)
+    // higher moments span too many orders of magnitude for a fixed-point
+    // grid; they use FP64 atomics (hit_accumulator.hpp:128-138)
+    const double v = (double)drop;
     const double v2 = v * v;
-    unsafeAtomicAdd(P.mom + 2 * (size_t)p, v2 * v);
-    unsafeAtomicAdd(P.mom + 2 * (size_t)p + 1, v2 * v * v);
+    unsafeAtomicAdd(P.mom + 2 * (size_t)a_slot, v2 * v);
+    unsafeAtomicAdd(P.mom + 2 * (size_t)a_slot + 1, v2 * v * v);
   }
   if (ADJ) {
     // derivative weights can exceed 1 (by about k s / (1 - s)); they use a
     // scale 2^8 smaller than the primal sums
-    const double sa = (double)scale * (1.0 / 256.0);
-    atomicAdd(P.adj + 2 * (size_t)p, (unsigned long long)__double2ll_rn((double)dds * sa));
-    atomicAdd(P.adj + 2 * (size_t)p + 1, (unsigned long long)__double2ll_rn((double)ddn * sa));
+    const float sa = scale * (1.0f / 256.0f);
+    atomicAdd(P.adj + 2 * (size_t)a_slot, (unsigned long long)__float2ll_rn(dds * sa));
+    atomicAdd(P.adj + 2 * (size_t)a_slot + 1, (unsigned long long)__float2ll_rn(ddn * sa));
   }
 }
 
@@ -587,7 +595,9 @@ __device__ __forceinline__ f3 disc_origin(const inst_desc& D, uint64_t id, uint3
 __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_state& R) {
   R.inst = find_instance(P, gid);
   const inst_desc& D = P.desc[R.inst];
-  R.id = gid - D.ray_begin;
+  // the ray's id within its instance's run (the Philox counter), so that a
+  // run of rays [b, b + n) traces exactly those rays of a larger run
+  R.id = gid - D.ray_begin + D.ray_id0;
   const u4 u = philox({(uint32_t)R.id, (uint32_t)(R.id >> 32), 0u, 0u}, D.seed, D.key1);
   if (D.source == 0) {
     R.o = make(fmaf(D.xmax - D.xmin, u01(u.x), D.xmin), fmaf(D.ymax - D.ymin, u01(u.y), D.ymin), D.src_z);
@@ -682,8 +692,9 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
     dds = g * R.rfac;
     ddn = drop * R.score;
   }
-  const float scale = D.fx_scale;
-  deposit<ADJ, FP64ACC, COUNT>(P, prim, drop, scale, dds, ddn, C);
+  const float scale = D.fx_scale, scale2 = D.fx_scale2;
+  const int32_t acc_off = D.acc_off;
+  deposit<ADJ, FP64ACC, COUNT>(P, (uint32_t)((int32_t)prim + acc_off), drop, scale, scale2, dds, ddn, C);
   if (disc) {  // check_for_additional_intersections (tracer.hpp:339-383)
     // the neighbour row starts at the w word of this record's last float4
     // (loaded with the normal) and ends at that of the next record; the
@@ -700,7 +711,8 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
         if (q + u >= q1) break;
         if (COUNT) C.nbt++;
         const float4* nr = P.prims + ((size_t)j[u] << P.prim_shift);
-        if (local_intersect(R.o, R.d, nr[0], nr[1])) deposit<ADJ, FP64ACC, COUNT>(P, j[u], drop, scale, dds, ddn, C);
+        if (local_intersect(R.o, R.d, nr[0], nr[1]))
+          deposit<ADJ, FP64ACC, COUNT>(P, (uint32_t)((int32_t)j[u] + acc_off), drop, scale, scale2, dds, ddn, C);
       }
     }
   }

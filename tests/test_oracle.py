@@ -404,3 +404,319 @@ def test_tie_rank_orders_equal_distance_hits():
 
 def small_trench_scene():
     return geometry.trench_discs(width=1.0, depth=2.0, flat=0.5, ylen=0.5, spacing=0.05)
+
+
+# ---------------------------------------------------------------------------
+# Mirror reflection (specular::use, src/rti/reflection/specular.hpp:16-40).
+# ---------------------------------------------------------------------------
+def test_specular_direction_invariants():
+    """d' = d - 2 (d.n) n: |d'| = |d|, d'.n = -d.n, d' - d parallel to n,
+    the tangential part unchanged, and d' = H d with the Householder matrix
+    H = I - 2 n n^T evaluated in FP64."""
+    rng = np.random.default_rng(21)
+    for _ in range(2000):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        n32, d32 = n.astype(np.float32), d.astype(np.float32)
+        r = oracle.specular_dir(d32, n32).astype(np.float64)
+        nn, dd = n32.astype(np.float64), d32.astype(np.float64)
+        assert np.linalg.norm(r) == pytest.approx(np.linalg.norm(dd), rel=1e-6)
+        assert float(r @ nn) == pytest.approx(-float(dd @ nn), abs=1e-6)
+        diff = r - dd
+        assert np.linalg.norm(np.cross(diff, nn)) < 1e-6
+        tang = dd - (dd @ nn) * nn
+        assert np.allclose(r - (r @ nn) * nn, tang, atol=1e-6)
+        house = np.eye(3) - 2.0 * np.outer(nn, nn)
+        assert np.allclose(r, house @ dd, atol=1e-6)
+    # axis-aligned mirrors flip exactly one component, bit for bit
+    d = np.array([0.3, -0.4, -0.8660254], dtype=np.float32)
+    for ax in range(3):
+        n = np.zeros(3, dtype=np.float32)
+        n[ax] = 1.0
+        want = d.copy()
+        want[ax] = -want[ax]
+        assert np.array_equal(oracle.specular_dir(d, n), want)
+
+
+def _mirror_kind(scene):
+    """Surface kind of each triangle of the mirror trench: 0 flat, 1 wall, 2 floor."""
+    from tests.helpers import MIRROR
+    c = scene["verts"][scene["tris"]].mean(1)
+    kind = np.ones(c.shape[0], dtype=np.int64)
+    kind[c[:, 2] < 1e-6] = 2
+    kind[(np.abs(c[:, 2] - MIRROR["depth"]) < 1e-6) & (np.abs(c[:, 0]) > MIRROR["width"] / 2)] = 0
+    return kind
+
+
+def _mirror_setup(n_rays, seed, s=0.3):
+    from tests import helpers as H
+    sc = H.mirror_trench()
+    box, sz = geometry.scene_box(sc, H.MIRROR["src_z"])
+    org, dirn = oracle.source_rays(box, sz, n_rays, seed=seed)
+    img = H.mirror_images(org, dirn, s)
+    return sc, box, sz, img
+
+
+def _singular_rays(img, n_rays, delta=1e-3):
+    """Rays with a hit within delta (arc length) of a corner of the profile:
+    there a hit and the next one can lie closer than tnear, which the tracer
+    skips by design (tracer.hpp:179), so the straight-line count does not
+    apply."""
+    ray, arc = img[0], img[2]
+    corners = np.array([0.5, 2.5, 3.5, 5.5])
+    near = np.min(np.abs(arc[:, None] - corners[None, :]), axis=1) < delta
+    sing = np.zeros(n_rays, dtype=bool)
+    sing[ray[near]] = True
+    return sing
+
+
+def test_mirror_trench_method_of_images_per_ray():
+    """All-mirror trench without rejection control: every path is a straight
+    line through the mirror images, so the number of hits of each ray, and
+    what it deposits on the flats, walls and floor, follow from the source
+    ray alone (computed in FP64 in tests/helpers.py, independently of the
+    oracle's reflection and closest-hit code)."""
+    n_rays, s = 2000, 0.3
+    sc, box, sz, img = _mirror_setup(n_rays, seed=11, s=s)
+    ray, k, arc, y, dep, kind_hit, _ = img
+    sing = _singular_rays(img, n_rays)
+    assert sing.sum() < 0.01 * n_rays
+    kind_tri = _mirror_kind(sc)
+    checked = 0
+    for i in range(n_rays):
+        if sing[i]:
+            continue
+        r = oracle.trace(sc, n_rays=1, ray_begin=i, box=box, src_z=sz, sticking=s, bc_x=0, bc_y=1, diffuse_prob=0.0,
+                         roulette=False, seed=11)
+        mine = ray == i
+        assert int(r["cnt"].sum()) == int(mine.sum()), f"ray {i}: hit count"
+        assert r["surface_hits"] == int(mine.sum())
+        for kk in range(3):
+            want = float(dep[mine & (kind_hit == kk)].sum())
+            got = float(r["s1"][kind_tri == kk].sum())
+            assert got == pytest.approx(want, rel=1e-5, abs=1e-30), f"ray {i}: deposit on surface kind {kk}"
+        checked += 1
+    assert checked > 0.99 * n_rays
+    # the geometry makes most rays reflect off the walls
+    assert np.mean(np.bincount(ray, minlength=n_rays)) > 2.0
+
+
+def test_mirror_trench_method_of_images_per_quad():
+    """The same at 5e4 rays per quad of the mesh: sums and hit totals within
+    0.2 % (rays that pass within tnear of a corner lose their later hits)."""
+    from tests import helpers as H
+    n_rays, s = 50000, 0.3
+    sc, box, sz, img = _mirror_setup(n_rays, seed=12, s=s)
+    _, _, arc, y, dep, _, _ = img
+    q, nq, nseg = H.mirror_quads(sc)
+    want_s, want_c = H.mirror_quad_sums(arc, y, dep, nq, nseg)
+    r = oracle.trace(sc, n_rays=n_rays, box=box, src_z=sz, sticking=s, bc_x=0, bc_y=1, diffuse_prob=0.0,
+                     roulette=False, seed=12)
+    got_s = np.bincount(q, weights=r["s1"], minlength=nq)
+    got_c = np.bincount(q, weights=r["cnt"].astype(np.float64), minlength=nq)
+    assert abs(got_c.sum() - want_c.sum()) <= 2e-3 * want_c.sum()
+    assert np.abs(got_s - want_s).sum() <= 2e-3 * want_s.sum()
+    assert np.all(np.abs(got_c - want_c) <= 20)
+
+
+# ---------------------------------------------------------------------------
+# Closest hit (rtcIntersect1 semantics, tracer.hpp:176-181: t in [tnear, tfar],
+# no back-face culling; boundary faces boundary_x_y.hpp:147-235): the oracle's
+# BVH traversal against brute force over every primitive and the four faces.
+# ---------------------------------------------------------------------------
+def _brute_closest(scene, box, org, dirn, tnear):
+    """All primitives and faces per ray, FP32 arithmetic in the order of
+    the oracle's expressions (its build has FP contraction off): returns
+    (prim, face, t) as oracle.closest_hits does."""
+    f = np.float32
+    o = org.astype(f)
+    d = dirn.astype(f)
+    n = o.shape[0]
+    tnear = f(tnear)
+    # boundary faces, in the order 0 xmax, 2 xmin, 1 ymin, 3 ymax; later ties win
+    best = np.full(n, np.finfo(f).max, dtype=f)
+    face = np.full(n, -1, dtype=np.int32)
+    xmin, xmax, ymin, ymax, zmin, zmax = (f(v) for v in box)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for fid, plane, ax, other, lo, hi in ((0, xmax, 0, 1, ymin, ymax), (2, xmin, 0, 1, ymin, ymax),
+                                              (1, ymin, 1, 0, xmin, xmax), (3, ymax, 1, 0, xmin, xmax)):
+            ok_dir = d[:, ax] != 0
+            t = (plane - o[:, ax]) / d[:, ax]
+            a = o[:, other] + d[:, other] * t
+            z = o[:, 2] + d[:, 2] * t
+            ok = ok_dir & (t >= tnear) & (t <= best) & (a >= lo) & (a <= hi) & (z >= zmin) & (z <= zmax)
+            best = np.where(ok, t, best)
+            face = np.where(ok, fid, face)
+        if scene["kind"] == "discs":
+            c = scene["xyzr"][:, :3].astype(f)
+            rad = scene["xyzr"][:, 3].astype(f)
+            nr = scene["normals"].astype(f)
+            dn = (d[:, None, 0] * nr[None, :, 0] + d[:, None, 1] * nr[None, :, 1]) + d[:, None, 2] * nr[None, :, 2]
+            co = c[None, :, :] - o[:, None, :]
+            num = (co[..., 0] * nr[None, :, 0] + co[..., 1] * nr[None, :, 1]) + co[..., 2] * nr[None, :, 2]
+            th = num / dn
+            hx = (o[:, None, 0] + d[:, None, 0] * th) - c[None, :, 0]
+            hy = (o[:, None, 1] + d[:, None, 1] * th) - c[None, :, 1]
+            hz = (o[:, None, 2] + d[:, None, 2] * th) - c[None, :, 2]
+            inside = ((hx * hx + hy * hy) + hz * hz) < (rad * rad)[None, :]
+            valid = (dn != 0) & (th >= tnear) & (th <= best[:, None]) & inside
+        else:
+            v = scene["verts"].astype(f)
+            t3 = scene["tris"]
+            v0, v1, v2 = v[t3[:, 0]], v[t3[:, 1]], v[t3[:, 2]]
+            e1, e2 = v1 - v0, v2 - v0
+
+            def cross(a, b):
+                return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                                 a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                                 a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1)
+
+            def dot(a, b):
+                return (a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1]) + a[..., 2] * b[..., 2]
+
+            pv = cross(d[:, None, :], e2[None, :, :])
+            det = dot(e1[None, :, :], pv)
+            idet = f(1) / det
+            tv = o[:, None, :] - v0[None, :, :]
+            u = dot(tv, pv) * idet
+            qv = cross(tv, e1[None, :, :])
+            vv = dot(d[:, None, :], qv) * idet
+            th = dot(e2[None, :, :], qv) * idet
+            valid = ((det != 0) & (u >= 0) & (u <= 1) & (vv >= 0) & ((u + vv) <= 1) & (th >= tnear)
+                     & (th <= best[:, None]))
+    thv = np.where(valid, th, np.inf).astype(np.float64)
+    prim = np.argmin(thv, axis=1).astype(np.int32)  # first (lowest index) of equal minima
+    hit = np.isfinite(thv[np.arange(n), prim])
+    t = np.where(hit, th[np.arange(n), prim], best).astype(f)
+    return np.where(hit, prim, -1), np.where(hit, -1, face), t
+
+
+def _random_rays(rng, n, lo, hi):
+    org = rng.uniform(lo, hi, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def test_closest_hit_matches_brute_force_discs():
+    """Overlapping coplanar discs (exact distance ties: the lower index
+    wins), tilted and downward-facing discs (back faces are returned), rays
+    from above, below and inside the box in every direction."""
+    rng = np.random.default_rng(31)
+    g = np.arange(-0.25, 0.2501, 0.1)
+    X, Y = np.meshgrid(g, g)
+    grid = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    rnd = rng.uniform([-0.45, -0.45, -0.4], [0.45, 0.45, 0.4], size=(40, 3))
+    xyz = np.concatenate([grid, rnd])
+    nrm = np.concatenate([np.tile([0.0, 0.0, 1.0], (grid.shape[0], 1)), rng.normal(size=(40, 3))])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    rad = np.concatenate([np.full(grid.shape[0], 0.1 * math.sqrt(2) / 2 + 0.01), rng.uniform(0.02, 0.12, 40)])
+    sc = {"kind": "discs", "xyzr": np.c_[xyz, rad].astype(np.float32), "normals": nrm.astype(np.float32)}
+    box = np.array([-0.5, 0.5, -0.5, 0.5, -0.5, 0.6], dtype=np.float32)
+    org, d = _random_rays(rng, 4000, [-0.49, -0.49, -0.45], [0.49, 0.49, 0.55])
+    # straight down onto the grid: many exact ties between overlapping discs
+    org2 = np.c_[rng.uniform(-0.3, 0.3, (1000, 2)), np.full(1000, 0.5)].astype(np.float32)
+    d2 = np.tile(np.array([0.0, 0.0, -1.0], dtype=np.float32), (1000, 1))
+    org, d = np.concatenate([org, org2]), np.concatenate([d, d2])
+    p, fc, t = oracle.closest_hits(sc, box, org, d, tnear=1e-4)
+    bp, bf, bt = _brute_closest(sc, box, org, d, 1e-4)
+    assert np.array_equal(p, bp)
+    assert np.array_equal(fc, bf)
+    assert np.array_equal(t, bt)
+    # the cases the test means to cover occur
+    hitn = sc["normals"][p[p >= 0]]
+    back = np.einsum("ij,ij->i", d[p >= 0], hitn) > 0
+    assert back.sum() > 100 and (~back).sum() > 100
+    assert (fc >= 0).sum() > 100
+    c = sc["xyzr"][:, :3]
+    exact_ties = 0
+    for i in np.nonzero(p[-1000:] >= 0)[0] + 4000:
+        hit_z = org[i, 2] + d[i, 2] * t[i]
+        on = np.hypot(c[: grid.shape[0], 0] - org[i, 0], c[: grid.shape[0], 1] - org[i, 1]) < rad[0]
+        exact_ties += int(on.sum() > 1 and abs(hit_z) < 1e-6)
+    assert exact_ties > 100
+
+
+def test_closest_hit_matches_brute_force_triangles():
+    """Random triangles, a duplicated triangle (exact ties), triangles that
+    share an edge, rays from everywhere in every direction."""
+    rng = np.random.default_rng(32)
+    v0 = rng.uniform(-0.45, 0.45, size=(60, 3))
+    e1 = rng.normal(scale=0.15, size=(60, 3))
+    e2 = rng.normal(scale=0.15, size=(60, 3))
+    verts = np.concatenate([v0, v0 + e1, v0 + e2])
+    tris = np.stack([np.arange(60), np.arange(60) + 60, np.arange(60) + 120], 1)
+    # a horizontal square of two triangles sharing an edge, and a copy of the first
+    sq = np.array([[-0.3, -0.3, 0.05], [0.3, -0.3, 0.05], [0.3, 0.3, 0.05], [-0.3, 0.3, 0.05]])
+    base = verts.shape[0]
+    verts = np.concatenate([verts, sq])
+    tris = np.concatenate([tris, [[base, base + 1, base + 2], [base, base + 2, base + 3], [base, base + 1, base + 2]]])
+    sc = {"kind": "triangles", "verts": verts.astype(np.float32), "tris": tris.astype(np.uint32)}
+    box = np.array([-0.5, 0.5, -0.5, 0.5, -0.6, 0.6], dtype=np.float32)
+    org, d = _random_rays(rng, 4000, [-0.49, -0.49, -0.55], [0.49, 0.49, 0.55])
+    org2 = np.c_[rng.uniform(-0.29, 0.29, (600, 2)), np.full(600, 0.5)].astype(np.float32)
+    d2 = np.tile(np.array([0.0, 0.0, -1.0], dtype=np.float32), (600, 1))
+    org, d = np.concatenate([org, org2]), np.concatenate([d, d2])
+    p, fc, t = oracle.closest_hits(sc, box, org, d, tnear=1e-4)
+    bp, bf, bt = _brute_closest(sc, box, org, d, 1e-4)
+    assert np.array_equal(p, bp)
+    assert np.array_equal(fc, bf)
+    assert np.array_equal(t, bt)
+    # rays through the duplicated triangle take the lower index, never the copy
+    assert np.sum(p == 60) > 50 and np.sum(p == 62) == 0
+    assert (p >= 0).sum() > 1000 and (fc >= 0).sum() > 500
+
+
+def test_closest_hit_matches_closed_form():
+    """The FP32 disc and triangle distances against the exact ray-plane
+    intersection in FP64."""
+    rng = np.random.default_rng(33)
+    for _ in range(500):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        c = rng.uniform(-0.2, 0.2, 3)
+        o = c + rng.normal(size=3) * 0.3 + n * 0.5
+        tgt = c + np.cross(n, rng.normal(size=3)) * 0.01
+        d = (tgt - o) / np.linalg.norm(tgt - o)
+        sc = {"kind": "discs", "xyzr": np.array([[*c, 0.2]], dtype=np.float32),
+              "normals": n[None].astype(np.float32)}
+        box = np.array([-5, 5, -5, 5, -5, 5], dtype=np.float32)
+        p, _, t = oracle.closest_hits(sc, box, o[None], d[None], tnear=1e-4)
+        o64, d64, c64, n64 = (x.astype(np.float32).astype(np.float64) for x in (o, d, c, n))
+        want = (c64 - o64) @ n64 / (d64 @ n64)
+        assert p[0] == 0
+        assert float(t[0]) == pytest.approx(want, rel=2e-5)
+
+
+# ---------------------------------------------------------------------------
+# Relative error and variance of the variance against the compiled reference
+# hit_accumulator (src/rti/trace/hit_accumulator.hpp:128-249).
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed,n_prims,n_dep", [(41, 7, 50), (42, 40, 5000), (43, 3, 200000)])
+def test_error_stats_match_compiled_reference(seed, n_prims, n_dep):
+    L = _ref()
+    rng = np.random.default_rng(seed)
+    ids = rng.integers(0, n_prims, n_dep).astype(np.uint32)
+    ids[ids == n_prims - 1] = 0  # one primitive without deposits: undefined statistics
+    vals = (rng.random(n_dep) * 0.9 + 0.05).astype(np.float32)
+    s1 = np.zeros(n_prims)
+    cnt = np.zeros(n_prims, dtype=np.uint64)
+    rel_ref = np.zeros(n_prims)
+    vov_ref = np.zeros(n_prims)
+    L.ref_hit_accumulator(n_prims, ids.ctypes.data_as(_u32p), vals.ctypes.data_as(_fp), n_dep,
+                          s1.ctypes.data_as(oracle._dp), cnt.ctypes.data_as(_u64p), rel_ref.ctypes.data_as(oracle._dp),
+                          vov_ref.ctypes.data_as(oracle._dp))
+    v = vals.astype(np.float64)
+    sums = [np.bincount(ids, weights=v ** k, minlength=n_prims) for k in (1, 2, 3, 4)]
+    mycnt = np.bincount(ids, minlength=n_prims).astype(np.uint64)
+    assert np.array_equal(mycnt, cnt)
+    np.testing.assert_allclose(sums[0], s1, rtol=1e-12)
+    rel, vov = oracle.error_stats(sums[0], sums[1], mycnt, sums[2], sums[3], total=n_dep)
+    big = np.finfo(np.float64).max
+    empty = mycnt == 0
+    assert np.all(rel[empty] == big) and np.all(rel_ref[empty] == big)
+    assert np.all(vov[empty] == big) and np.all(vov_ref[empty] == big)
+    np.testing.assert_allclose(rel[~empty], rel_ref[~empty], rtol=1e-9)
+    np.testing.assert_allclose(vov[~empty], vov_ref[~empty], rtol=1e-6, atol=1e-12)
