@@ -174,6 +174,12 @@ typedef struct rti_instance {
 
 void rti_default_params(rti_params* p);
 
+/* No reference counterpart: the device's architecture name (e.g. "gfx950")
+ * and compute-unit count, for tools that key measurements by device (the
+ * autotuner's cache). name may be NULL; it receives at most cap bytes,
+ * NUL-terminated. */
+int rti_device_info(int device, char* name, uint64_t cap, int* num_cus);
+
 int rti_ctx_create(rti_ctx** out, int device);
 int rti_ctx_destroy(rti_ctx* ctx);
 const char* rti_last_error(const rti_ctx* ctx);

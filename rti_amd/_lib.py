@@ -39,7 +39,7 @@ EXPORTS = [
     "rti_get_hierarchy", "rti_get_wide_hierarchy",
     "rti_batch_set", "rti_batch_run", "rti_batch_replay", "rti_batch_get_flux", "rti_batch_get_counts",
     "rti_batch_get_stats", "rti_get_error_stats", "rti_batch_get_error_stats", "rti_batch_get_adjoint",
-    "rti_trace_noalloc",
+    "rti_trace_noalloc", "rti_device_info",
 ]
 
 
@@ -152,10 +152,21 @@ def _declare(L):
     L.rti_batch_get_error_stats.argtypes = [vp, c.c_uint32, _dp, _dp]
     L.rti_batch_get_adjoint.argtypes = [vp, c.c_uint32, _dp, _dp]
     L.rti_trace_noalloc.argtypes = [vp, c.c_void_p, c.c_uint64]
+    L.rti_device_info.argtypes = [c.c_int, c.c_char_p, c.c_uint64, c.POINTER(c.c_int)]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("rti_default_params", "rti_last_error"):
             f.restype = c.c_int
+
+
+def device_info(device=0):
+    """(architecture name, compute units) of a HIP device."""
+    buf = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    rc = lib().rti_device_info(int(device), buf, 64, ctypes.byref(cus))
+    if rc != RTI_OK:
+        raise RtiError(rc, "rti_device_info failed (no HIP device?)")
+    return buf.value.decode(errors="replace"), int(cus.value)
 
 
 def default_params(**kw):

@@ -19,6 +19,7 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -111,7 +112,10 @@ inline bool next_tag(const std::string& s, size_t& pos, xml_tag& t, size_t limit
 class b64_reader {
  public:
   b64_reader(const std::string& s, size_t pos) : s_(s), pos_(pos) {}
+  // Most bytes the rest of the text can still decode to.
+  size_t max_remaining() const { return (size_t)have_ + (s_.size() - std::min(pos_, s_.size())) / 4 * 3 + 3; }
   std::vector<unsigned char> read(size_t nbytes) {
+    if (nbytes > max_remaining()) throw std::runtime_error("base64 data ends early");
     std::vector<unsigned char> out;
     out.reserve(nbytes);
     while (out.size() < nbytes) {
@@ -172,9 +176,11 @@ struct byte_source {
   size_t pos = 0;
   bool base64 = true;
   b64_reader* b64 = nullptr;
+  // Most bytes the source can still deliver.
+  size_t max_remaining() const { return base64 ? b64->max_remaining() : s->size() - std::min(pos, s->size()); }
   std::vector<unsigned char> read(size_t n) {
     if (base64) return b64->read(n);
-    if (pos + n > s->size()) throw std::runtime_error("raw appended data ends early");
+    if (pos > s->size() || n > s->size() - pos) throw std::runtime_error("raw appended data ends early");
     std::vector<unsigned char> out(s->begin() + pos, s->begin() + pos + n);
     pos += n;
     return out;
@@ -220,6 +226,8 @@ inline std::vector<unsigned char> read_binary_block(byte_source& src, int hsize,
   }
   auto h0 = src.read(hsize);
   const uint64_t nblocks = read_uint(h0.data(), hsize);
+  // bound the block count by what the data can hold before sizing anything by it
+  if (nblocks > src.max_remaining() / (uint64_t)hsize) throw std::runtime_error("zlib header ends early");
   auto h1 = src.read((2 + nblocks) * hsize);
   const uint64_t bsize = read_uint(h1.data(), hsize);
   const uint64_t last = read_uint(h1.data() + hsize, hsize);

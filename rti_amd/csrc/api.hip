@@ -915,6 +915,20 @@ void rti_default_params(rti_params* p) {
   p->accum = RTI_ACCUM_FIXED64;
 }
 
+int rti_device_info(int device, char* name, uint64_t cap, int* num_cus) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RTI_ERR_HIP;
+  if (device < 0 || device >= ndev) return RTI_ERR_INVALID_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RTI_ERR_HIP;
+  if (name && cap > 0) {
+    std::strncpy(name, prop.gcnArchName, cap - 1);
+    name[cap - 1] = '\0';
+  }
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return RTI_OK;
+}
+
 int rti_ctx_create(rti_ctx** out, int device) {
   if (!out) return RTI_ERR_INVALID_ARG;
   *out = nullptr;

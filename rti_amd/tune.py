@@ -16,7 +16,9 @@ import hashlib
 import json
 import os
 
-from ._lib import Context
+import numpy as np
+
+from ._lib import Context, device_info
 
 COMPACT = (40, 48, 52, 56, 60, 64)
 LDS = (4, 6, 8, 16)
@@ -27,13 +29,25 @@ WAVES = (0, 20, 24)
 MIN_GAIN = 0.005  # relative gain a coordinate step must show to change a knob
 
 
-def signature(items):
+def signature(items, device=None):
+    """Cache key of a workload on a device: every item's geometry (a digest
+    of its arrays), ray budget, physics, boundaries and source, and the
+    device's architecture and compute-unit count."""
     h = hashlib.sha1()
+    if device is None:
+        device = device_info(0)
+    h.update(f"{device[0]}:{device[1]}".encode())
+    digests = {}
     for it in items:
         sc = it["scene"]
-        arr = sc["xyzr"] if sc["kind"] == "discs" else sc["tris"]
-        h.update(f"{sc['kind']}:{arr.shape[0]}:{it.get('n_rays')}:{it.get('sticking')}:"
-                 f"{it.get('cos_exponent', 1.0)}:{it.get('diffuse_prob', 1.0)}".encode())
+        if id(sc) not in digests:
+            g = hashlib.sha1()
+            for key in (("xyzr", "normals") if sc["kind"] == "discs" else ("verts", "tris")):
+                g.update(np.ascontiguousarray(sc[key]).tobytes())
+            digests[id(sc)] = g.hexdigest()
+        h.update(f"{sc['kind']}:{digests[id(sc)]}:{it.get('n_rays')}:{it.get('sticking')}:"
+                 f"{it.get('cos_exponent', 1.0)}:{it.get('diffuse_prob', 1.0)}:{it.get('bc_x', 0)}:"
+                 f"{it.get('bc_y', 0)}:{it.get('source', 0)}:{it.get('src_z')}".encode())
     return h.hexdigest()[:16]
 
 
@@ -58,7 +72,7 @@ def default_fraction(items, sample_rays=2e8, full_up_to=2e8):
 
 def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size, leaf_batch, refill_lanes} and their rate."""
-    key = signature(items)
+    key = signature(items) if cache else None
     if cache and os.path.exists(cache):
         with open(cache) as fh:
             db = json.load(fh)
@@ -92,7 +106,10 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
             cur = scores.get(best[name])  # None: the starting value is not a candidate
             if cur is None or scores[top] > cur * (1.0 + MIN_GAIN):
                 best[name] = top
-    out = dict(best, segments_per_s=max(t["segments_per_s"] for t in trials), trials=trials, signature=key)
+        # the rate of the knobs returned (the descent can keep a starting
+        # value that another trial beat by less than MIN_GAIN)
+        rate = _rate(ctx, best)
+    out = dict(best, segments_per_s=rate, trials=trials, signature=key)
     if cache:
         db = {}
         if os.path.exists(cache):

@@ -257,6 +257,23 @@ def test_tuner_descent_ignores_gains_within_noise(monkeypatch):
     assert k["waves_per_cu"] == 0
     assert k["lds_stack"] == 4
     assert k["leaf_batch"] == 48
+    # the reported rate is that of the knobs returned
+    assert out["segments_per_s"] == rate(None, k)
     # every knob was visited: six knobs, block size and leaf threshold twice
     assert len(out["trials"]) == (len(tune.COMPACT) + len(tune.WAVES) + len(tune.LDS) + 2 * len(tune.BLOCK)
                                   + 2 * len(tune.LEAF) + len(tune.REFILL))
+
+
+def test_tuner_cache_key_covers_geometry_physics_and_device():
+    """The tuner's cache key changes with the geometry's contents (same
+    size), the boundary conditions, the source and the device."""
+    from rti_amd import tune
+    sc = {"kind": "discs", "xyzr": np.ones((5, 4), np.float32), "normals": np.ones((5, 3), np.float32)}
+    sc2 = {"kind": "discs", "xyzr": np.ones((5, 4), np.float32) * 2, "normals": np.ones((5, 3), np.float32)}
+    dev = ("gfx950", 256)
+    base = tune.signature([{"scene": sc, "n_rays": 10}], device=dev)
+    assert base == tune.signature([{"scene": sc, "n_rays": 10}], device=dev)
+    for other in ([{"scene": sc2, "n_rays": 10}], [{"scene": sc, "n_rays": 10, "bc_x": 1}],
+                  [{"scene": sc, "n_rays": 10, "source": 1}], [{"scene": sc, "n_rays": 10, "src_z": 3.0}]):
+        assert tune.signature(other, device=dev) != base
+    assert tune.signature([{"scene": sc, "n_rays": 10}], device=("gfx942", 304)) != base

@@ -8,6 +8,7 @@ headers; uncompressed or zlib blocks. The files here are written by a small
 generator that follows the VTK XML file-format layout.
 """
 import os
+import struct
 import subprocess
 import xml.etree.ElementTree as ET
 
@@ -154,3 +155,21 @@ def test_writer_layout_and_round_trip(tmp_path):
     codes = [int(x) for x in fd.text.split()]
     assert fd.attrib["Name"] == "cmd" and codes[-1] == 0
     assert bytes(codes[:-1]).decode() == 'probe <&> "quoted"'
+
+
+@pytest.mark.parametrize("compress", [False, True])
+def test_corrupt_binary_headers_are_errors(tmp_path, compress):
+    """A byte count or block count near 2^64 in an appended raw array (it
+    would wrap the bounds arithmetic) is reported as truncated data, not
+    read past the buffer."""
+    pts, nrm, rad = cloud(seed=6)
+    path = tmp_path / "in.vtp"
+    write_vtk_cloud(str(path), pts, nrm, rad, fmt="appended", header="UInt64", encoding="raw", compress=compress)
+    data = bytearray(path.read_bytes())
+    start = data.index(b"_", data.index(b"<AppendedData")) + 1
+    # the first array's header: byte count (plain) or block count (zlib)
+    data[start:start + 8] = struct.pack("<Q", 0xFFFFFFFFFFFFFFF8)
+    path.write_bytes(bytes(data))
+    r = subprocess.run([PROBE, str(path)], capture_output=True, text=True)
+    assert r.returncode == 1, (r.returncode, r.stderr)
+    assert "ends early" in r.stderr
