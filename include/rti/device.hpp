@@ -30,6 +30,7 @@
 
 #include <array>
 #include <cmath>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -73,7 +74,11 @@ class device final {
       throw std::invalid_argument("rti::device: points, normals and grid spacing differ in length");
     if (points.empty()) throw std::invalid_argument("rti::device: no points");
     if (std::isnan((double)cosExponent))
-      throw std::invalid_argument("rti::device: the source direction is not a cos^n distribution about -z");
+      throw std::invalid_argument(
+          "rti::device: the source direction is not a cos^n distribution about -z (a direction class must "
+          "override cos_exponent())");
+    // host-side plugin evaluation first: no device work for an invalid particle
+    const numeric_type sticking = sticking_probability();
     if (!ctx) check(rti_ctx_create(&ctx, 0), "rti_ctx_create");
     const size_t n = points.size();
     std::vector<float> xyzr(4 * n), nrm(3 * n);
@@ -92,7 +97,7 @@ class device final {
     p.bc_y = yCond == bound_condition::PERIODIC ? RTI_BC_PERIODIC : RTI_BC_REFLECTIVE;
     p.cos_exponent = (float)cosExponent;
     p.source = circularSource ? RTI_SOURCE_DISC : RTI_SOURCE_RECTANGLE;
-    p.sticking = (float)sticking_probability();
+    p.sticking = (float)sticking;
     p.diffuse_prob = std::is_base_of<reflection::diffuse<numeric_type>, reflection_type>::value ? 1.0f : 0.0f;
     check(rti_set_params(ctx, &p), "rti_set_params");
     check(rti_run(ctx), "rti_run");
@@ -108,14 +113,31 @@ class device final {
   std::vector<size_t> get_hit_cnts() { return hitcnts; }
 
  private:
+  // The disc cloud as the particle sees it (point_cloud_disc_geometry's
+  // meta_geometry view, src/rti/geo/point_cloud_disc_geometry.hpp:40-101):
+  // normals by primitive id, a reference into the stored normals, and print
+  // in the reference's "(:class ...)" form with one "(x y z r)" per disc.
+  struct disc_cloud_geometry : geo::meta_geometry<numeric_type> {
+    const std::vector<util::triple<numeric_type>>* pts = nullptr;
+    std::vector<util::triple<numeric_type>>* nrm = nullptr;
+    const std::vector<numeric_type>* rad = nullptr;
+    void print(std::ostream& os) override {
+      os << "(:class rti::device::disc_cloud_geometry";
+      for (size_t i = 0; i < pts->size(); ++i)
+        os << " (" << (*pts)[i][0] << " " << (*pts)[i][1] << " " << (*pts)[i][2] << " " << (*rad)[i] << ")";
+      os << ")";
+    }
+    util::triple<numeric_type> get_normal(unsigned int primID) override { return nrm->at(primID); }
+    util::triple<numeric_type>& get_normal_ref(unsigned int primID) override { return nrm->at(primID); }
+  };
+
   // The particle's sticking probability, queried once with a hit at the
   // first disc (user particles see a valid ray, hit and geometry).
   numeric_type sticking_probability() {
-    struct one_disc_geometry : geo::meta_geometry<numeric_type> {
-      util::triple<numeric_type> nrm;
-      util::triple<numeric_type> get_normal(unsigned int) override { return nrm; }
-    } geom;
-    geom.nrm = normals[0];
+    disc_cloud_geometry geom;
+    geom.pts = &points;
+    geom.nrm = &normals;
+    geom.rad = &spacing;
     RTCRay ray{};
     ray.org_x = (float)points[0][0];
     ray.org_y = (float)points[0][1];

@@ -5,7 +5,7 @@
 //   rti::geo::bound_condition            src/rti/geo/bound_condition.hpp:4-6
 //   rti::rng::i_rng (+ i_state)          src/rti/rng/i_rng.hpp:11-30
 //   rti::rng::mt64_rng                   src/rti/rng/mt64_rng.hpp:8-50
-//   rti::geo::meta_geometry              src/rti/geo/meta_geometry.hpp:8-25
+//   rti::geo::meta_geometry              src/rti/geo/meta_geometry.hpp:8-25 (no Embree handles)
 //   rti::particle::i_particle            src/rti/particle/i_particle.hpp:8-16
 //   rti::reflection::i_reflection        src/rti/reflection/i_reflection.hpp:7-19
 //   rti::reflection::diffuse / specular  src/rti/reflection/diffuse.hpp, specular.hpp
@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <limits>
 #include <memory>
+#include <ostream>
 #include <random>
 #include <vector>
 
@@ -55,11 +56,17 @@ using quadruple = std::array<T, 4>;
 namespace geo {
 enum class bound_condition { REFLECTIVE, PERIODIC };
 
+// The geometry a particle or reflection sees (meta_geometry.hpp:8-25). The
+// reference's get_rtc_device / get_rtc_geometry return Embree handles and
+// have no counterpart without Embree; print, get_normal, get_normal_ref and
+// get_new_origin keep their signatures.
 template <typename T>
 class meta_geometry {
  public:
   virtual ~meta_geometry() {}
+  virtual void print(std::ostream& os) = 0;
   virtual util::triple<T> get_normal(unsigned int primID) = 0;
+  virtual util::triple<T>& get_normal_ref(unsigned int primID) = 0;
   virtual util::triple<T> get_new_origin(RTCRay& r, unsigned int) {
     return {(T)(r.org_x + r.dir_x * r.tfar), (T)(r.org_y + r.dir_y * r.tfar), (T)(r.org_z + r.dir_z * r.tfar)};
   }

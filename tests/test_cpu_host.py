@@ -87,6 +87,33 @@ def test_dropin_header_compiles_and_links(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+def _plugins_exe():
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_plugins")
+    if not os.path.exists(exe):
+        subprocess.run(["bash", os.path.join(ROOT, "tests", "cpp", "build.sh")], check=True)
+    return exe
+
+
+def test_dropin_refuses_a_direction_without_cos_exponent():
+    """A user direction class with only get() (the reference's i_direction,
+    src/rti/ray/i_direction.hpp:9-19) compiles against the drop-in; run()
+    refuses it with std::invalid_argument naming cos_exponent(), before the
+    particle or the device is touched (no GPU needed)."""
+    r = subprocess.run([_plugins_exe(), "direction"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cos_exponent" in r.stdout
+
+
+def test_dropin_particle_sees_the_disc_cloud_geometry():
+    """A user particle calls geometry.get_normal_ref(hit.primID) (a reference
+    into the stored normals, equal to get_normal) and geometry.print(os)
+    (src/rti/geo/meta_geometry.hpp:12,16); an invalid probability it returns
+    is refused before any device work (no GPU needed)."""
+    r = subprocess.run([_plugins_exe(), "particle"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ref_ok 1 ref_is_stored 1 print_ok 1" in r.stdout and "sticking probability" in r.stdout
+
+
 def test_dropin_rejects_unsupported_reflection_at_compile_time(tmp_path):
     code = """
 #include "rti/device.hpp"

@@ -47,6 +47,16 @@ def test_cpp_dropin_matches_c_abi(tmp_path, circular):
     np.testing.assert_array_equal(data[:, 8].astype(np.uint64), cnt)
 
 
+def test_cpp_dropin_plugin_particle_runs():
+    """The drop-in with a user particle that reads the geometry through
+    get_normal_ref and print: a full run on the GPU."""
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_plugins")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    _say("drop-in plugin particle on the GPU")
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def test_invalid_inputs_return_error_codes():
     sc = small_trench()
     with Context() as ctx:
