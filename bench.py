@@ -12,8 +12,8 @@ Workloads (``--workload``):
          s = 0.1, 1e6 rays (the reference's CPU-runnable case).
   batch  64 disc-cloud trenches and holes (aspect 2-20, 1e4-5e5 discs,
          s log-uniform 1e-3..1, diffuse or specular), ray budgets
-         proportional to area; ``--rays`` is the batch total (6.4e7 here;
-         the generator's own budgets sum to 6.4e8).
+         proportional to area, 1e6-5e7 per instance (6.4e8 in total);
+         ``--rays`` rescales the batch total.
   rough  periodic rough surface over 16 holes of aspect 6 (about 4M discs),
          70/30 diffuse/specular, s = 0.02, ``--rays`` (1e9); also reports
          64-bit fixed point vs FP64 atomic accumulation on the same paths.
@@ -78,13 +78,13 @@ def workload(name, rays):
                 f"periodic y, source z=4.1, cosine, s=0.1, {int(r)} rays")
         return cases, desc, 32.0
     if name == "batch":
-        r = 6.4e7 if rays is None else rays
-        inst = geometry.batch_instances(seed=3, count=64, total_rays=r)
+        inst = geometry.batch_instances(seed=3, count=64, total_rays=rays)
+        r = sum(i["n_rays"] for i in inst)
         cases = [dict(scene=i["scene"], n_rays=i["n_rays"], sticking=i["sticking"], diffuse_prob=i["diffuse_prob"],
                       bc_x=1, bc_y=1, seed=3) for i in inst]
         nd = sum(c["scene"]["xyzr"].shape[0] for c in cases)
         desc = (f"64 disc-cloud trenches/holes (AR 2-20, {nd} discs total, s log-uniform 1e-3..1, diffuse or "
-                f"specular), {int(r)} rays total proportional to area")
+                f"specular), {int(r)} rays total proportional to area (1e6-5e7 per instance)")
         return cases, desc, 32.0
     if name == "adjoint":
         r = 1e8 if rays is None else rays

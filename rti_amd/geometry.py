@@ -283,13 +283,13 @@ def rough_surface_discs(seed=5, nfield=4096, cells=4, hole_radius=0.15, aspect=6
     cx = rng.uniform(0.0, L, n_bumps)
     cy = rng.uniform(0.0, L, n_bumps)
     g = (np.arange(nfield, dtype=np.float64) + 0.5) * (L / nfield)
-    field = np.zeros((nfield, nfield), dtype=np.float32)
-    for a, w, x0, y0 in zip(amp, wid, cx, cy):
-        dx = (g - x0 + L / 2) % L - L / 2
-        dy = (g - y0 + L / 2) % L - L / 2
-        gx = np.exp(-0.5 * (dx / w) ** 2).astype(np.float32)
-        gy = np.exp(-0.5 * (dy / w) ** 2).astype(np.float32)
-        field += np.float32(a) * np.outer(gy, gx)
+    # each bump is separable, a(b) gy_b(y) gx_b(x): the field is one matrix
+    # product GY^T diag(a) GX (FP64, rounded once to FP32)
+    dx = (g[None, :] - cx[:, None] + L / 2) % L - L / 2
+    dy = (g[None, :] - cy[:, None] + L / 2) % L - L / 2
+    gx = np.exp(-0.5 * (dx / wid[:, None]) ** 2)
+    gy = np.exp(-0.5 * (dy / wid[:, None]) ** 2)
+    field = ((gy * amp[:, None]).T @ gx).astype(np.float32)
     # gradient (periodic central differences)
     hstep = L / nfield
     fx = (np.roll(field, -1, 1) - np.roll(field, 1, 1)) / np.float32(2 * hstep)
