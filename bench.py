@@ -103,24 +103,48 @@ def workload(name, rays):
     raise SystemExit(f"unknown workload {name}")
 
 
-def bytes_per_segment(counters, prim_bytes, deposit_bytes=48.0):
-    """Algorithmic bytes per segment from an instrumented run.
-
-    64 B per node visit (one node record, binary or four-wide), ``prim_bytes``
-    per primitive test (triangle 48 B, disc 32 B), 36 B per neighbour test
-    (disc record + column index), ``deposit_bytes`` per deposit (read + write
-    of the 8-byte accumulators: three, or five with the adjoint sums), 24 B
-    per front hit (unit normal + neighbour row pointers).
-    """
-    seg, hits = float(counters[0]), float(counters[1])
+def bytes_per_segment(counters):
+    """Algorithmic bytes per segment from an instrumented run, with the
+    per-unit weights of the measurement plan:
+        B_seg = 32 V_node + 32 V_prim + 24 D + 32 K_nb + 4 K_nb + 64
+    V_node: tree nodes visited, V_prim: primitives tested, D: deposits,
+    K_nb: neighbour discs tested (disc record + column index), 64: the ray
+    state read and written once per segment. The traversal here visits
+    four-wide nodes; each visit counts as one node at 32 B (one visit tests
+    four children's boxes, stored quantised in a 64-byte record)."""
+    seg = float(counters[0])
     nodes, prims, nbt, deps = (float(x) for x in counters[4:8])
-    total = 64.0 * nodes + prim_bytes * prims + 36.0 * nbt + deposit_bytes * deps + 24.0 * hits
-    return total / seg, {"V_node": nodes / seg, "V_prim": prims / seg, "K_nb": nbt / seg, "D": deps / seg}
+    parts = {"V_node": nodes / seg, "V_prim": prims / seg, "K_nb": nbt / seg, "D": deps / seg}
+    b = 32.0 * parts["V_node"] + 32.0 * parts["V_prim"] + 24.0 * parts["D"] + 36.0 * parts["K_nb"] + 64.0
+    return b, parts
 
 
-def cpu_baseline(cases, budget_s=20.0, adjoint=False):
-    """Oracle (restated reference tracer, OpenMP) on a bounded ray sample:
-    a pilot of 2e4 rays per case sizes the sample to about budget_s seconds."""
+def record_bytes_per_segment(parts, prim_bytes, deposit_bytes):
+    """The same visits priced at the sizes of this layout's records: 64 B per
+    four-wide node, prim_bytes per primitive test (48 B triangle, 32 B disc),
+    36 B per neighbour test, deposit_bytes per deposit (read + write of the
+    8-byte sums: three, five with the adjoint), no ray-state traffic (rays
+    stay in registers)."""
+    return 64.0 * parts["V_node"] + prim_bytes * parts["V_prim"] + 36.0 * parts["K_nb"] + deposit_bytes * parts["D"]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cases, budget_s=7.0, runs=3, adjoint=False):
+    """Oracle (restated reference tracer, OpenMP on every host core) on a
+    bounded ray sample of the same cases, sized by a 2e4-ray pilot to about
+    budget_s seconds per run; the trace region only (the reference times
+    tracer.hpp:122-286, without the BVH build); the median of ``runs`` runs
+    after the pilot warmed up."""
     import oracle
     from rti_amd import geometry
     threads = os.cpu_count() or 1
@@ -140,11 +164,16 @@ def cpu_baseline(cases, budget_s=20.0, adjoint=False):
     pilot = 20000
     segs, secs = run(pilot)
     per_case = int(pilot * min(2000.0, max(1.0, budget_s / max(secs, 1e-3))))
-    if per_case > pilot:
+    rates = []
+    for _ in range(runs):
         segs, secs = run(per_case)
-    return {"value": segs / secs, "unit": "segments/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (restated reference tracer, own BVH, OpenMP) on {per_case} rays for each of "
-                      f"{len(cases)} cases of the same geometry: {segs} segments in {secs:.1f} s"}
+        rates.append(segs / secs)
+    value = float(np.median(rates))
+    return {"value": value, "unit": "segments/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "runs": [round(r, 1) for r in rates],
+            "sample": f"oracle (restated reference tracer, own BVH, OpenMP, {threads} threads) on {per_case} rays "
+                      f"for each of {len(cases)} cases of the same geometry, {segs} segments per run; median of "
+                      f"{runs} runs"}
 
 
 def radiosity_report(ctx, cases):
@@ -225,11 +254,12 @@ def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
 
 def profile_figure(name, key):
     """A figure of the committed counter summary of the trace kernel on this
-    workload (a line "key: value" or "... key = value"), or None without one.
-    Keys: "bytes_per_segment" (HBM-side traffic from FETCH_SIZE and WRITE_SIZE,
-    each scaled by what the counter reads for a copy kernel of known size),
-    "l2_hit_rate" (TCC_HIT / (TCC_HIT + TCC_MISS)), "/ busy cycles"
-    (SQ_ACTIVE_INST_VALU over the SIMDs' busy cycles)."""
+    workload: the number after "key:" at the start of a line, or None without
+    one. Keys: "bytes_per_segment" (HBM-side traffic from FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
+    known size), "l2_hit_rate" (TCC_HIT / (TCC_HIT + TCC_MISS)), "valu_busy"
+    (SQ_ACTIVE_INST_VALU over the SIMDs' busy cycles), "active_lanes"
+    (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64)."""
     path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
     if not os.path.exists(path):
         return None
@@ -237,8 +267,6 @@ def profile_figure(name, key):
         for line in fh:
             if line.startswith(key + ":"):
                 return float(line.split(":")[1].split()[0])
-            if key.startswith("/") and key in line and "=" in line:
-                return float(line.split("=")[-1])
     return None
 
 
@@ -329,21 +357,36 @@ def main():
     cnt_ctx.batch_set(small)
     cnt_ctx.set_params(kernel=2, **knobs)
     cnt_ctx.batch_run()
-    B, parts = bytes_per_segment(cnt_ctx.counters(), prim_bytes, 80.0 if adj else 48.0)
+    B, parts = bytes_per_segment(cnt_ctx.counters())
     cnt_ctx.close()
+    B_rec = record_bytes_per_segment(parts, prim_bytes, 80.0 if adj else 48.0)
     achieved = value * B / 1e9
     b_meas = profile_figure(args.workload, "bytes_per_segment")
     traffic = None if b_meas is None else b_meas * segments
+    traffic_gbs = None if traffic is None else traffic / (ms_per_step / 1e3) / 1e9
+    valu_busy = profile_figure(args.workload, "valu_busy")
+    lanes = profile_figure(args.workload, "active_lanes")
+    # the resource that binds, from the committed counters of this workload:
+    # VALU issue when the SIMDs are nearly always issuing while HBM-side
+    # traffic stays well below the peak, HBM otherwise
+    binding = None
+    if valu_busy is not None and traffic_gbs is not None:
+        binding = "valu" if valu_busy >= 0.85 and traffic_gbs < 0.7 * HBM_PEAK_GBS else "hbm"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_gbs": None if traffic is None else round(traffic / (ms_per_step / 1e3) / 1e9, 1),
+                "traffic_gbs": None if traffic_gbs is None else round(traffic_gbs, 1),
+                "traffic_frac": None if traffic_gbs is None else round(traffic_gbs / HBM_PEAK_GBS, 4),
                 "l2_hit_rate": profile_figure(args.workload, "l2_hit_rate"),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
-                "valu_busy": profile_figure(args.workload, "/ busy cycles"),
-                "note": "achieved and frac: algorithmic bytes per segment x segments/s over the HBM peak; "
-                        "traffic, traffic_gbs, l2_hit_rate and valu_busy: from the committed counter summary "
-                        "of the trace kernel on this workload (null without one). A valu_busy near 1 with "
-                        "traffic_gbs well below the peak means VALU issue, not HBM, binds the kernel"}
+                "bytes_per_segment_records": round(B_rec, 1),
+                "binding": binding, "valu_busy": valu_busy, "active_lanes": lanes,
+                "useful_valu_frac": None if (valu_busy is None or lanes is None) else round(valu_busy * lanes / 64, 3),
+                "note": "achieved and frac: B_seg = 32 V_node + 32 V_prim + 24 D + 36 K_nb + 64 bytes per segment "
+                        "(V_node four-wide node visits) x segments/s over the HBM peak; bytes_per_segment_records "
+                        "prices the same visits at this layout's record sizes. traffic, traffic_gbs, l2_hit_rate, "
+                        "valu_busy (VALU issue over busy SIMD cycles) and active_lanes (of 64, per issued VALU "
+                        "instruction) come from the committed counter summary of the trace kernel on this "
+                        "workload (null without one); binding names the resource those counters show binding"}
 
     cpu = None
     if not args.no_cpu:
