@@ -176,13 +176,20 @@ def cpu_baseline(cases, budget_s=7.0, runs=3, adjoint=False):
                       f"{runs} runs"}
 
 
-def radiosity_report(ctx, cases):
+def radiosity_report(cases, knobs, shards=20):
     """Hole workload: the mean incident flux on 20 wall bands (depth 0.25
-    each) of every (s, n) case of the last run, against the FP64 radiosity
-    solution of the same hole committed in tests/golden/radiosity_hole_bands.json.
-    Incident flux of a triangle = raw / (s * area) * source area / rays; band
-    sigma from the per-triangle sigmas; bound per band: 3 sigma + 1 % (the
-    320-gon mesh against the true cylinder)."""
+    each) of every (s, n) case, against the FP64 radiosity solution of the
+    same hole integrated over the same wall triangles (committed in
+    tests/golden/radiosity_hole_bands.json). Incident flux of a triangle =
+    raw / (s * area) * source area / rays. The cases' rays are traced as
+    ``shards`` consecutive ray ranges of equal size (the same rays as one
+    run; integer sums add exactly), and a band's standard error is the spread
+    of its shard means: a history that deposits several times in one band
+    (most at small s) makes the per-deposit variance too small. Bound per
+    band: 3 standard errors + 0.2 % (the 320-gon facets against the
+    cylinder move the flux by about 1e-4; the solution's discretisation
+    by less)."""
+    from rti_amd import Context
     with open(os.path.join(ROOT, "tests", "golden", "radiosity_hole_bands.json")) as fh:
         ref = json.load(fh)
     sc = cases[0]["scene"]
@@ -195,27 +202,40 @@ def radiosity_report(ctx, cases):
     lo, hi = sc["verts"].min(0), sc["verts"].max(0)
     a_src = float((hi[0] - lo[0]) * (hi[1] - lo[1]))
     edges = ref["band_edges_depth"]
+    nb = len(edges) - 1
+    masks = [wall & (depth >= edges[b]) & (depth < edges[b + 1]) for b in range(nb)]
+    n = cases[0]["n_rays"]
+    assert all(cs["n_rays"] == n for cs in cases)
+    per = n // shards
+    est = np.zeros((shards, len(cases), nb))
+    with Context(0) as ctx:
+        for k in range(shards):
+            ctx.batch_set([dict(cs, n_rays=per, ray_begin=k * per, rays_total=per * shards) for cs in cases])
+            ctx.set_params(**knobs)
+            ctx.batch_run()
+            for i, cs in enumerate(cases):
+                g = ctx.batch_flux(i)["raw"] / (cs["sticking"] * area) * a_src / per
+                for b in range(nb):
+                    w = area[masks[b]]
+                    est[k, i, b] = np.sum(g[masks[b]] * w) / w.sum()
     out, ok = [], True
-    for k, case in enumerate(cases):
-        s, n_exp, n = case["sticking"], case.get("cos_exponent", 1.0), case["n_rays"]
-        want = next(r["wall_band_flux"] for r in ref["cases"] if r["sticking"] == s and r["cos_exponent"] == n_exp)
-        f = ctx.batch_flux(k)
-        g = f["raw"] / (s * area) * a_src / n
-        rel, zs, within = [], [], True
-        for b in range(len(want)):
-            m = wall & (depth >= edges[b]) & (depth < edges[b + 1])
-            w = area[m]
-            est = float(np.sum(g[m] * w) / w.sum())
-            sig = float(np.sqrt(np.sum((f["sigma"][m] / s * a_src / n) ** 2)) / w.sum())
-            err = abs(est - want[b])
-            rel.append(err / want[b])
-            zs.append(err / sig if sig > 0 else float("inf"))
-            within = within and err < 3 * sig + 0.01 * want[b]
+    for i, cs in enumerate(cases):
+        s, n_exp = cs["sticking"], cs.get("cos_exponent", 1.0)
+        want = np.array(next(r["wall_band_flux"] for r in ref["cases"] if r["sticking"] == s
+                             and r["cos_exponent"] == n_exp))
+        m = est[:, i, :].mean(0)
+        se = est[:, i, :].std(0, ddof=1) / math.sqrt(shards)
+        err = np.abs(m - want)
+        within = bool(np.all(err < 3 * se + 0.002 * want))
         ok = ok and within
-        out.append({"s": s, "n": n_exp, "max_rel_err": round(max(rel), 5), "mean_rel_err": round(float(np.mean(rel)), 5),
-                    "max_err_over_sigma": round(max(zs), 2), "within_3sigma_plus_1pct": within})
-    return {"reference": "FP64 radiosity of the same hole (tests/golden/radiosity_hole_bands.json)",
-            "bands": "20 wall bands, depth 0.25", "cases": out, "all_within": ok}
+        out.append({"s": s, "n": n_exp, "max_rel_err": round(float((err / want).max()), 5),
+                    "mean_rel_err": round(float((err / want).mean()), 5),
+                    "max_err_over_sigma": round(float((err / se).max()), 2),
+                    "within_3sigma_plus_0.2pct": within})
+    return {"reference": "FP64 radiosity of the same hole over the same wall triangles "
+                         "(tests/golden/radiosity_hole_bands.json)",
+            "bands": f"20 wall bands, depth 0.25; standard errors from {shards} ray-range shards of {per} rays",
+            "cases": out, "all_within": ok}
 
 
 def adjoint_fd_check(case, knobs, ds=1e-3, dn=1e-2, seeds=8):
@@ -318,9 +338,8 @@ def main():
 
     extra = {}
     if args.workload == "hole":
-        # flux of the last timed step against the FP64 radiosity solution
-        ctx.batch_run()
-        extra["flux_vs_radiosity"] = radiosity_report(ctx, cases)
+        # the same rays, in shards, against the FP64 radiosity solution
+        extra["flux_vs_radiosity"] = radiosity_report(cases, knobs)
         log(f"flux vs radiosity: {extra['flux_vs_radiosity']}")
     if args.workload == "rough":
         # same paths, FP64 atomic accumulation: the fixed point's error

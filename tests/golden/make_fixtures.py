@@ -18,9 +18,9 @@ JSON files).
 * radiosity_hole_bands.json: the FP64 radiosity solution (oracle/radiosity.py,
   itself pinned by radiosity_ar45.json) of the benchmark's hole, radius 0.5,
   depth 5, for s in {0.01, 0.1, 1} x n in {1, 10}: the incident flux on 20
-  wall bands of depth 0.25 (means of 20 radiosity elements each), which the
-  benchmark compares its full-size fluxes with. Needs no reference tree
-  (``python tests/golden/make_fixtures.py radiosity_hole_bands.json``).
+  wall bands of depth 0.25, averaged over the benchmark mesh's wall
+  triangles as the benchmark averages its full-size fluxes. Needs no
+  reference tree (``python tests/golden/make_fixtures.py radiosity_hole_bands.json``).
 
 The inputs are transcribed from the reference test sources; the expected
 values are those the reference asserts.
@@ -150,16 +150,19 @@ def hole_bands():
     import sys
     sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     from oracle import radiosity as rad
-    out = {"source": "oracle/radiosity.py solve_hole (FP64 radiosity, ring-to-ring view factors, cos^n source)",
-           "geometry": "cylindrical hole radius 0.5, depth 5, opening at z = 5",
-           "quantity": "mean incident flux per wall band relative to the source flux density",
-           "band_edges_depth": None, "cases": []}
+    from rti_amd import geometry
+    sc = geometry.hole_triangles()
+    edges = np.linspace(0.0, 5.0, 21)
+    out = {"source": "oracle/radiosity.py hole_mesh_band_flux (FP64 radiosity of the cylinder, 2000 wall "
+                     "elements and 20 floor annuli, cos^n source, integrated over the mesh's wall triangles)",
+           "geometry": "rti_amd.geometry.hole_triangles(): radius 0.5, depth 5, 320 x 256 wall facets",
+           "quantity": "area-weighted mean incident flux over the wall triangles whose centroid depth lies in "
+                       "each band, relative to the source flux density",
+           "band_edges_depth": edges.tolist(), "cases": []}
     for s in (0.01, 0.1, 1.0):
         for n in (1.0, 10.0):
-            ref = rad.solve_hole(0.5, 5.0, s, cos_exponent=n, n_wall=400, n_floor=10)
-            out["band_edges_depth"] = ref["z"][::20].tolist()
-            out["cases"].append({"sticking": s, "cos_exponent": n,
-                                 "wall_band_flux": ref["wall"].reshape(20, 20).mean(1).tolist()})
+            flux = rad.hole_mesh_band_flux(sc["verts"], sc["tris"], 0.5, 5.0, s, n, edges)
+            out["cases"].append({"sticking": s, "cos_exponent": n, "wall_band_flux": flux.tolist()})
     return out
 
 

@@ -213,10 +213,29 @@ def test_hole_band_fixture_matches_the_solver():
         fx = json.load(fh)
     assert [(c["sticking"], c["cos_exponent"]) for c in fx["cases"]] == [
         (s, n) for s in (0.01, 0.1, 1.0) for n in (1.0, 10.0)]
+    sc = geometry.hole_triangles()
+    edges = np.array(fx["band_edges_depth"])
+    np.testing.assert_allclose(edges, np.linspace(0.0, 5.0, 21), rtol=0, atol=1e-12)
     for case in fx["cases"][2:4]:
-        ref = radiosity.solve_hole(0.5, 5.0, case["sticking"], cos_exponent=case["cos_exponent"], n_wall=400, n_floor=10)
-        np.testing.assert_allclose(ref["wall"].reshape(20, 20).mean(1), case["wall_band_flux"], rtol=1e-12)
-        np.testing.assert_allclose(ref["z"][::20], fx["band_edges_depth"], rtol=0, atol=1e-12)
+        ref = radiosity.hole_mesh_band_flux(sc["verts"], sc["tris"], 0.5, 5.0, case["sticking"],
+                                            case["cos_exponent"], edges)
+        np.testing.assert_allclose(ref, case["wall_band_flux"], rtol=1e-12)
+
+
+def test_mesh_band_flux_quadrature():
+    """The band means over a mesh's triangles: a linear flux profile gives
+    the area-weighted mean of its values at the triangle centroids exactly,
+    and on a mesh whose rows line up with the bands the band means equal the
+    solution's own band means up to its discretisation."""
+    p = radiosity._triangle_points(8)
+    assert p.shape == (64, 2) and np.all(p.sum(1) < 1)
+    np.testing.assert_allclose(p.mean(0), [1 / 3, 1 / 3], atol=1e-15)
+    sc = geometry.hole_triangles(n_az=64, n_ax=40, n_floor=4, n_top=8)  # rows of 0.125: 2 per band
+    edges = np.linspace(0.0, 5.0, 21)
+    got = radiosity.hole_mesh_band_flux(sc["verts"], sc["tris"], 0.5, 5.0, 1.0, 1.0, edges)
+    ref = radiosity.solve_hole(0.5, 5.0, 1.0, n_wall=2000, n_floor=20)
+    want = ref["wall"].reshape(20, 100).mean(1)
+    np.testing.assert_allclose(got[1:], want[1:], rtol=3e-4)
 
 
 def test_radiosity_energy_balance():

@@ -283,35 +283,41 @@ def _incident(g, n_rays, s):
 @pytest.mark.parametrize("s,nexp", [(1.0, 1.0), (0.1, 1.0), (0.01, 1.0), (0.1, 10.0)])
 def test_hole_mesh_flux_matches_radiosity(s, nexp):
     """Triangle-mesh hole (radius 0.5, depth 5, 2 x 2 periodic cell): mean
-    incident flux per wall band and floor annulus vs the FP64 radiosity
-    solution of the same cylinder; bound: 3 sigma of the band mean + 1%
-    (the 320-gon mesh vs the true cylinder, and band averaging)."""
+    incident flux per wall band vs the FP64 radiosity solution of the same
+    cylinder integrated over the same wall triangles. The 2e7 rays run as 20
+    ray-range shards, whose spread gives each band's standard error; bound:
+    3 standard errors + 0.2 % (the 320-gon facets move the flux by ~1e-4)."""
     sc = geometry.hole_triangles(n_az=320, n_ax=100, n_floor=8, n_top=10)
-    n = 20_000_000
-    _say(f"hole mesh vs radiosity s={s} n={nexp}")
-    g = run_scene(sc, n_rays=n, sticking=s, cos_exponent=nexp, bc_x=1, bc_y=1, seed=77)
-    G = _incident(g, n, s)
-    v = sc["verts"][sc["tris"]].astype(np.float64)
-    c = v.mean(1)
-    depth = 5.0 - c[:, 2]
-    r = np.hypot(c[:, 0], c[:, 1])
-    wall = (c[:, 2] > 1e-6) & (c[:, 2] < 5.0 - 1e-6) & (np.abs(r - 0.5 * math.cos(math.pi / 320)) < 0.01)
-    # fine radiosity discretisation (20 elements per test band), averaged per band
-    ref = radiosity.solve_hole(0.5, 5.0, s, cos_exponent=nexp, n_wall=400, n_floor=10)
-    band_ref = ref["wall"].reshape(20, 20).mean(1)
-    edges = ref["z"][::20]
-    worst = 0.0
-    for k in range(20):
-        m = wall & (depth >= edges[k]) & (depth < edges[k + 1])
-        w = g["areas"][m]
-        est = np.sum(G[m] * w) / w.sum()
-        sig = np.sqrt(np.sum((g["sigma"][m] / s * 4.0 / n) ** 2)) / w.sum()
-        err = abs(est - band_ref[k])
-        worst = max(worst, err / band_ref[k])
-        assert err < 3 * sig + 0.01 * band_ref[k], (k, est, band_ref[k], sig)
-    top = c[:, 2] > 5.0 - 1e-6
-    np.testing.assert_allclose(np.average(G[top], weights=g["areas"][top]), 1.0, rtol=0.01)
-    _say(f"worst band relative error {worst:.4f}")
+    n, shards = 20_000_000, 20
+    per = n // shards
+    _say(f"hole mesh vs radiosity s={s} n={nexp}, {shards} shards of {per} rays")
+    edges = np.linspace(0.0, 5.0, 21)
+    want = radiosity.hole_mesh_band_flux(sc["verts"], sc["tris"], 0.5, 5.0, s, nexp, edges)
+    wall, depth = radiosity.hole_wall_mask(sc["verts"], sc["tris"], 0.5, 5.0)
+    est = np.zeros((shards, 20))
+    top_flux = []
+    with Context() as ctx:
+        ctx.set_scene(sc)
+        for k in range(shards):
+            ctx.set_params(n_rays=per, ray_begin=k * per, rays_total=n, sticking=s, cos_exponent=nexp, bc_x=1, bc_y=1,
+                           seed=77)
+            ctx.run()
+            f = ctx.flux()
+            f["areas"] = ctx.areas()
+            f["stats"] = ctx.stats()
+            G = _incident(f, per, s)
+            for b in range(20):
+                m = wall & (depth >= edges[b]) & (depth < edges[b + 1])
+                est[k, b] = np.average(G[m], weights=f["areas"][m])
+            top = depth <= 1e-6
+            top_flux.append(np.average(G[top], weights=f["areas"][top]))
+    m = est.mean(0)
+    se = est.std(0, ddof=1) / np.sqrt(shards)
+    err = np.abs(m - want)
+    _say(f"max |err| / se {np.max(err / se):.2f}, max rel err {np.max(err / want):.4f}")
+    assert np.all(err < 3 * se + 0.002 * want), list(zip(err / se, err / want))
+    # direct flux on the flat top is exactly the source's
+    np.testing.assert_allclose(np.mean(top_flux), 1.0, rtol=2e-3)
 
 
 def test_trench_disc_flux_matches_radiosity():
