@@ -22,6 +22,10 @@ JSON files).
   triangles as the benchmark averages its full-size fluxes. Needs no
   reference tree (``python tests/golden/make_fixtures.py radiosity_hole_bands.json``).
 
+* historical_mc_ar45.json: the reference's own recorded Monte Carlo profiles
+  of the aspect-45 cylinder (verification/radiosity/data/sp.{1,0.1.v2,0.01}.data),
+  shape only: 100 depth bins normalised to the first.
+
 The inputs are transcribed from the reference test sources; the expected
 values are those the reference asserts.
 """
@@ -166,10 +170,33 @@ def hole_bands():
     return out
 
 
+def historical_mc():
+    """The reference's recorded Monte Carlo profiles of the aspect-45 cylinder
+    (verification/radiosity/data/sp.1.data, sp.0.1.v2.data, sp.0.01.data,
+    written by an older rti with triangle_geometry_from_gmsh on
+    resources/cylinder/c45.msh): 100 depth bins (x = depth / 45), flux
+    normalised to the first bin, and the run lines (rays, hits, non-hits)."""
+    import re
+    base = os.path.join(REF, "verification", "radiosity", "data")
+    num = re.compile(r"^\s*[0-9.eE+-]+\s+[0-9.eE+-]+\s*$")
+    run = re.compile(r"(\d+)rays (\d+)hits (\d+)nonhits")
+    out = {"source": "rti verification/radiosity/data (BSD-3-Clause, Copyright (c) the rti authors)",
+           "geometry": "cylinder radius 0.5, depth 45 (resources/gmsh-based/cylinder/c45.geo:2), triangle mesh",
+           "quantity": "mean flux per depth bin of 0.45, divided by the first bin's", "profiles": {}}
+    for s, name in (("1", "sp.1.data"), ("0.1", "sp.0.1.v2.data"), ("0.01", "sp.0.01.data")):
+        text = open(os.path.join(base, name)).read().splitlines()
+        rows = [ln.split() for ln in text if num.match(ln)]
+        m = next(run.search(ln) for ln in text if run.search(ln))
+        out["profiles"][s] = {"file": name, "x": [float(r[0]) for r in rows], "flux": [float(r[1]) for r in rows],
+                              "rays": int(m.group(1)), "hits": int(m.group(2)), "nonhits": int(m.group(3))}
+    return out
+
+
 if __name__ == "__main__":
     import sys
     jobs = [("local_intersector_kats.json", local_intersector), ("area_inside_kats.json", area_inside),
-            ("radiosity_ar45.json", radiosity), ("radiosity_hole_bands.json", hole_bands)]
+            ("radiosity_ar45.json", radiosity), ("radiosity_hole_bands.json", hole_bands),
+            ("historical_mc_ar45.json", historical_mc)]
     if len(sys.argv) > 1:
         jobs = [j for j in jobs if j[0] in sys.argv[1:]]
     for name, fn in jobs:

@@ -323,3 +323,22 @@ def test_tuner_cache_key_covers_geometry_physics_and_device():
                   [{"scene": sc, "n_rays": 10, "source": 1}], [{"scene": sc, "n_rays": 10, "src_z": 3.0}]):
         assert tune.signature(other, device=dev) != base
     assert tune.signature([{"scene": sc, "n_rays": 10}], device=("gfx942", 304)) != base
+
+
+def test_historical_mc_profiles_against_radiosity():
+    """The reference's recorded Monte Carlo profiles of the aspect-45
+    cylinder (tests/golden/historical_mc_ar45.json, from an older rti) agree
+    in shape with the FP64 radiosity solution of the same cylinder, binned
+    the same way (100 bins of depth 0.45, divided by the first): within 5 %
+    over the upper 30 bins, within 15 % over the upper 90 (the deepest bins
+    hold few hits)."""
+    with open(os.path.join(ROOT, "tests", "golden", "historical_mc_ar45.json")) as fh:
+        fx = json.load(fh)
+    for s, p in fx["profiles"].items():
+        y = np.array(p["flux"])
+        assert y.size == 100 and y[0] == 1.0
+        ref = radiosity.solve_hole(0.5, 45.0, float(s), n_wall=4500, n_floor=20)
+        w = ref["wall"].reshape(100, 45).mean(1)
+        rel = np.abs(y - w / w[0]) / (w / w[0])
+        assert rel[:30].max() < 0.05, (s, rel[:30].max())
+        assert rel[:90].max() < 0.15, (s, rel[:90].max())
