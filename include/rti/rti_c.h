@@ -257,7 +257,9 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
  * power of two; nchild - 1 = (s_y < 0) | (s_z < 0) << 1; slots beyond
  * nchild hold an inverted box and a leaf of the node's subtree;
  * child >= 0 is a node index relative to this geometry, < 0 is ~stored
- * primitive position. Root is record 0. The reference's Embree BVH is
+ * primitive position. Root is record 0. Each family of siblings starts on
+ * an even record (a 128-byte line); records no node references are padding
+ * (an inverted box in all four slots). The reference's Embree BVH is
  * internal to Embree and has no such export; this one serves validation. */
 int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64_t* n_nodes);
 

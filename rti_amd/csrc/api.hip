@@ -69,7 +69,7 @@ struct instance {
   float ztop = 0;
   int share = -1;  // >= 0: this instance uses the geometry of instance `share` (same buffers)
   // derived at prepare
-  uint32_t prim_begin = 0, node_begin = 0;
+  uint32_t prim_begin = 0, node_begin = 0, node4_begin = 0;
   uint32_t acc_begin = 0;  // first accumulator slot
   uint64_t ray_begin = 0;
   float box[6] = {0, 0, 0, 0, 0, 0};
@@ -373,17 +373,20 @@ int prepare(rti_ctx* ctx) {
   // an instance that shares another's geometry takes that instance's
   // primitive records, trees and neighbour rows; every instance has its own
   // accumulator slots
-  uint64_t np = 0, nn = 0, maxn = 0, na = 0;
+  uint64_t np = 0, nn = 0, nn4 = 0, maxn = 0, na = 0;
   for (auto& I : ctx->inst) {
     if (I.share < 0) {
       I.prim_begin = (uint32_t)np;
       I.node_begin = (uint32_t)nn;
+      I.node4_begin = (uint32_t)nn4;  // even: each instance's families start on 128-byte lines
       np += I.n;
       nn += std::max<uint64_t>(I.n - 1, 1);
+      nn4 += rti_impl::wide_capacity((uint32_t)I.n);
       maxn = std::max(maxn, I.n);
     } else {
       I.prim_begin = ctx->inst[I.share].prim_begin;
       I.node_begin = ctx->inst[I.share].node_begin;
+      I.node4_begin = ctx->inst[I.share].node4_begin;
     }
     I.acc_begin = (uint32_t)std::min<uint64_t>(na, UINT32_MAX);
     na += I.n;
@@ -402,7 +405,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->prims.ensure(((np + 1) * 16) << ctx->prim_shift));
   HIP_TRY(ctx, ctx->perm.ensure(np * 4));
   HIP_TRY(ctx, ctx->nodes.ensure(nn * sizeof(bvh_node)));
-  HIP_TRY(ctx, ctx->nodes4.ensure(nn * sizeof(bvh4_node)));
+  HIP_TRY(ctx, ctx->nodes4.ensure(nn4 * sizeof(bvh4_node)));
   HIP_TRY(ctx, ctx->nb_ptr.ensure((np + 1) * 4));
   HIP_TRY(ctx, ctx->areas.ensure(np * 4));
   HIP_TRY(ctx, ctx->acc.ensure(na * 8 * rti_dev::kAccStride));
@@ -424,6 +427,7 @@ int prepare(rti_ctx* ctx) {
     J.n = (uint32_t)I.n;
     J.prim_begin = I.prim_begin;
     J.node_begin = I.node_begin;
+    J.node4_begin = I.node4_begin;
     centroid_bounds(I, J.cmin, J.cmax);
     J.rmax = I.rmax;
     J.area_box[0] = I.box[0];
@@ -556,7 +560,7 @@ int upload_descriptors(rti_ctx* ctx, inst_desc* dst) {
     d.bc_y = p.bc_y;
     d.prim_kind = I.kind;
     d.root = (int32_t)I.node_begin;
-    d.root4 = (int32_t)I.node_begin;
+    d.root4 = (int32_t)I.node4_begin;
     d.source = p.source;
     d.ray_begin = rb;
     d.ray_id0 = p.ray_begin;
@@ -1172,7 +1176,7 @@ int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64
   *n_nodes = I.n_wide;
   if (nodes) {
     if (cap_nodes < I.n_wide) return fail(ctx, RTI_ERR_INVALID_ARG, "node capacity too small");
-    HIP_TRY(ctx, hipMemcpy(nodes, ctx->nodes4.as<bvh4_node>() + I.node_begin, I.n_wide * sizeof(bvh4_node),
+    HIP_TRY(ctx, hipMemcpy(nodes, ctx->nodes4.as<bvh4_node>() + I.node4_begin, I.n_wide * sizeof(bvh4_node),
                            hipMemcpyDeviceToHost));
   }
   return RTI_OK;

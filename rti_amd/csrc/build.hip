@@ -476,7 +476,11 @@ __global__ void k_single_root(uint32_t prim_begin, uint32_t node_begin, bvh_node
 // fetches. Built level by level: a count pass gives each node's number of
 // internal children, a scan turns the counts into contiguous child slots
 // (siblings adjacent, breadth-first, deterministic), and a write pass stores
-// the quantised node and the next level's work items.
+// the quantised node and the next level's work items. Every family of
+// siblings starts on an even slot, i.e. on a 128-byte line (a family of odd
+// size is followed by one padding slot, and so is the root): a node and its
+// next sibling share the line a fetch brings into the cache. Padding slots
+// hold an inverted box in every slot and are never referenced.
 // ---------------------------------------------------------------------------
 struct c4_box {
   float lo[3], hi[3];
@@ -549,24 +553,43 @@ __device__ uint32_t c4_qhi(float v, float o, float s) {
   return (uint32_t)q;
 }
 
+// Internal children of each work item, rounded up to even (a padding slot
+// after a family of odd size); 0 for a padding item (< 0).
 __global__ void k_c4_count(const int* items, uint32_t width, const bvh_node* nodes, uint32_t* counts) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) counts[width] = 0;
   if (i >= width) return;
+  if (items[i] < 0) {
+    counts[i] = 0;
+    return;
+  }
   int ref[4];
   c4_box box[4];
   const int m = c4_members(nodes, items[i], ref, box);
   uint32_t k = 0;
   for (int c = 0; c < m; ++c) k += ref[c] >= 0 ? 1u : 0u;
-  counts[i] = k;
+  counts[i] = k + (k & 1u);
 }
 
 // items[i] (binary node) becomes four-wide slot node4_begin + base + i; its
 // internal children get slots node4_begin + base + width + offsets[i] + j.
+// A padding item (< 0) becomes a node whose four slots all hold an inverted
+// box and the leaf pad_leaf.
 __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, const bvh_node* nodes,
-                           const uint32_t* offsets, uint32_t node4_begin, bvh4_node* nodes4, int* next) {
+                           const uint32_t* offsets, uint32_t node4_begin, int pad_leaf, bvh4_node* nodes4,
+                           int* next) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= width) return;
+  if (items[i] < 0) {
+    bvh4_node pad;
+    pad.ox = pad.oy = pad.oz = 0.0f;
+    pad.sx = pad.sy = pad.sz = 1.0f;
+    pad.lo_x = pad.lo_y = pad.lo_z = 0xffffffffu;
+    pad.hi_x = pad.hi_y = pad.hi_z = 0u;
+    pad.child = make_int4(pad_leaf, pad_leaf, pad_leaf, pad_leaf);
+    nodes4[node4_begin + base + i] = pad;
+    return;
+  }
   int ref[4];
   c4_box box[4];
   const int m = c4_members(nodes, items[i], ref, box);
@@ -609,6 +632,7 @@ __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, cons
       child[c] = ref[c];
     }
   }
+  if (j & 1u) next[offsets[i] + j] = -1;  // the family's padding slot
   bvh4_node out;
   out.ox = lo[0];
   out.oy = lo[1];
@@ -649,7 +673,11 @@ __global__ void k_c4_stack(const bvh4_node* nodes4, uint32_t node4_begin, uint32
   need[base + i] = k ? (k - 1 + deepest) : 0u;
 }
 
-__global__ void k_c4_root(int* items, uint32_t node_begin) { items[0] = (int)node_begin; }
+// The root level: the binary root and one padding slot.
+__global__ void k_c4_root(int* items, uint32_t node_begin) {
+  items[0] = (int)node_begin;
+  items[1] = -1;
+}
 
 // ---------------------------------------------------------------------------
 // Disc-overlap graph. The pair predicate restates disc_neighborhood::check_dist
@@ -854,16 +882,18 @@ hipError_t scratch_reserve(build_scratch& s, uint32_t n) {
 #define RTI_ALLOC(ptr, bytes)                              \
   e = hipMalloc((void**)&(ptr), (bytes));                 \
   if (e != hipSuccess) { scratch_free(s); return e; }
-  RTI_ALLOC(s.keys[0], cap * 4ull);
-  RTI_ALLOC(s.keys[1], cap * 4ull);
-  RTI_ALLOC(s.vals[0], cap * 4ull);
-  RTI_ALLOC(s.vals[1], cap * 4ull);
+  // + 2: the four-wide collapse's work lists hold up to n items and counts
+  // one more (its levels include padding slots)
+  RTI_ALLOC(s.keys[0], (cap + 2ull) * 4ull);
+  RTI_ALLOC(s.keys[1], (cap + 2ull) * 4ull);
+  RTI_ALLOC(s.vals[0], (cap + 2ull) * 4ull);
+  RTI_ALLOC(s.vals[1], (cap + 2ull) * 4ull);
   RTI_ALLOC(s.hist, 256ull * nblocks * 4ull);
   size_t st = scan_tmp_size((uint32_t)(256 * nblocks));
   size_t st2 = scan_tmp_size(cap + 1);
   RTI_ALLOC(s.scan_tmp, (st > st2 ? st : st2) * 4ull);
   RTI_ALLOC(s.parent, (2ull * cap) * 4ull);
-  RTI_ALLOC(s.flags, cap * 4ull);
+  RTI_ALLOC(s.flags, (2ull * cap + 2ull) * 4ull);  // binary refit flags; four-wide stack needs
   RTI_ALLOC(s.leaf_lo, cap * 16ull);
   RTI_ALLOC(s.leaf_hi, cap * 16ull);
   RTI_ALLOC(s.counts, (cap + 1ull) * 4ull);
@@ -992,7 +1022,7 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
   int* items[2] = {reinterpret_cast<int*>(s.keys[0]), reinterpret_cast<int*>(s.keys[1])};
   k_c4_root<<<1, 1, 0, stream>>>(items[0], job.node_begin);
   std::vector<std::pair<uint32_t, uint32_t>> levels;  // (base, width)
-  uint32_t base = 0, width = 1;
+  uint32_t base = 0, width = 2;
   int cur = 0;
   hipError_t e;
   while (width > 0) {
@@ -1001,7 +1031,8 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
     e = exclusive_scan_u32(s.vals[0], s.vals[1], width + 1, s.scan_tmp, stream);
     if (e != hipSuccess) return e;
     k_c4_write<<<blocks_for(width, T), T, 0, stream>>>(items[cur], width, base, st.nodes, s.vals[1],
-                                                       job.node_begin, st.nodes4, items[cur ^ 1]);
+                                                       job.node4_begin, ~(int)job.prim_begin, st.nodes4,
+                                                       items[cur ^ 1]);
     uint32_t next = 0;
     e = hipMemcpyAsync(&next, s.vals[1] + width, 4, hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return e;
@@ -1010,10 +1041,10 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
     base += width;
     width = next;
     cur ^= 1;
-    if (base + width > std::max<uint32_t>(job.n - 1, 1)) return hipErrorInvalidValue;  // cannot happen
+    if (base + width > wide_capacity(job.n)) return hipErrorInvalidValue;  // cannot happen
   }
   for (size_t l = levels.size(); l-- > 0;)
-    k_c4_stack<<<blocks_for(levels[l].second, T), T, 0, stream>>>(st.nodes4, job.node_begin, levels[l].first,
+    k_c4_stack<<<blocks_for(levels[l].second, T), T, 0, stream>>>(st.nodes4, job.node4_begin, levels[l].first,
                                                                    levels[l].second, s.flags);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

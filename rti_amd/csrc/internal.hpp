@@ -17,6 +17,7 @@ struct build_job {
   uint32_t n;               // primitives
   uint32_t prim_begin;      // first absolute prim slot
   uint32_t node_begin;      // first absolute node slot (max(n - 1, 1) nodes)
+  uint32_t node4_begin;     // first absolute four-wide slot (wide_capacity(n) slots, even)
   const float* d_xyzr;      // discs, input order, n x 4
   const float* d_nrm;       // discs, n x 3
   const float* d_verts;     // triangles
@@ -59,6 +60,11 @@ struct build_scratch {
   uint32_t* lead = nullptr;       // n + 1: PLOC merge flags
 };
 
+// Four-wide slots an instance of n primitives may use: its nodes plus one
+// padding slot per family (each family starts on a 128-byte line) and the
+// root's padding slot.
+inline uint32_t wide_capacity(uint32_t n) { return 2u * (n > 2u ? n - 1u : 1u) + 2u; }
+
 hipError_t scratch_reserve(build_scratch& s, uint32_t n);
 void scratch_free(build_scratch& s);
 
@@ -67,7 +73,7 @@ void scratch_free(build_scratch& s);
 hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* depth_out,
                           hipStream_t stream);
 // Four-wide traversal tree collapsed from the instance's binary tree, into
-// nodes4[node_begin ..). n_wide: nodes written; stack_bound: the most stack
+// nodes4[node4_begin ..). n_wide: slots written; stack_bound: the most stack
 // entries a closest-hit traversal of it can hold.
 hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* n_wide,
                       uint32_t* stack_bound, hipStream_t stream);

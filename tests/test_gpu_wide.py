@@ -67,7 +67,19 @@ def _validate_wide(wide, perm, scene):
                 stack.append(ch)
             else:
                 seen_prim[~ch] += 1
-    assert np.all(seen_node == 1), "every four-wide node is reached exactly once"
+    reached = seen_node > 0
+    assert np.all(seen_node[reached] == 1), "every four-wide node is reached exactly once"
+    # the slots no node reaches are padding: an inverted box in all four slots
+    for v in np.nonzero(~reached)[0]:
+        assert all(int(wide["lo"][v][k]) == 0xFFFFFFFF and int(wide["hi"][v][k]) == 0 for k in range(3)), v
+    # every family of siblings starts on an even slot (a 128-byte line)
+    for v in order:
+        _, m = wide_scales(wide[v])
+        kids = [int(wide["child"][v][c]) for c in range(m) if int(wide["child"][v][c]) >= 0]
+        if kids:
+            assert min(kids) % 2 == 0, f"node {v}: family at odd slot {min(kids)}"
+            assert kids == list(range(min(kids), min(kids) + len(kids))), "siblings are adjacent"
+    assert (~reached).sum() <= reached.sum() + 1
     assert np.all(seen_prim == 1), f"{np.sum(seen_prim != 1)} primitives are not exactly one leaf"
     sub_lo = np.full((len(wide), 3), np.inf)
     sub_hi = np.full((len(wide), 3), -np.inf)
@@ -124,7 +136,8 @@ def test_wide_tree_encloses_every_primitive(which, builder):
         wide = ctx.wide_hierarchy()
         st = ctx.stats()
     assert st["wide_nodes"] == len(wide)
-    assert len(wide) <= max(n - 1, 1)
+    # at most n - 1 nodes plus one padding record per family of siblings
+    assert len(wide) <= 2 * max(n - 1, 1) + 2
     assert 0 <= st["wide_stack"] <= 64
     _validate_wide(wide, perm, sc)
 
