@@ -116,10 +116,12 @@ typedef struct rti_params {
   int32_t moments;        /* 1 = also accumulate sums of deposit^3 and deposit^4 (FP64)
                              for the variance of the variance                          */
   int32_t source;         /* rti_source (default RTI_SOURCE_RECTANGLE)                  */
-  int32_t leaf_batch;     /* persistent kernel, four-wide tree: leaf tests are queued per
-                             lane and run in wave-wide leaf phases once this many 64ths
-                             of the wave's tracing lanes hold a queued leaf (1..64);
-                             0 = default (48 when the store holds triangles, else -1);
+  int32_t leaf_batch;     /* persistent kernel, four-wide tree: hit leaves go on a list
+                             shared by the wave's lanes and are tested in leaf phases, up
+                             to 64 at a time, one per lane; a leaf phase runs once the
+                             entries it would take reach this many 64ths of the lanes
+                             with a node left (1..256), or no node is left;
+                             0 = default (128 when the store holds triangles, else -1);
                              -1 = test leaves at each node visit                       */
   int32_t tree_builder;   /* rti_tree_builder; 0 = default (PLOC for triangle meshes,
                              LBVH for disc clouds)                                      */

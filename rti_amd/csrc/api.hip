@@ -201,7 +201,7 @@ int check_params(rti_ctx* ctx, const rti_params& p) {
     return fail(ctx, RTI_ERR_INVALID_ARG, "lds_stack must be 4, 6, 8, 16, 24 or 32");
   if (p.waves_per_cu < 0 || p.waves_per_cu > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "waves_per_cu out of range");
   if (p.compact_lanes < 0 || p.compact_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "compact_lanes out of range");
-  if (p.leaf_batch < -1 || p.leaf_batch > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..64");
+  if (p.leaf_batch < -1 || p.leaf_batch > 256) return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch must be -1 or 0..256");
   if (p.refill_lanes < 0 || p.refill_lanes > 64) return fail(ctx, RTI_ERR_INVALID_ARG, "refill_lanes must be 0..64");
   if (p.tree_builder != 0 && p.tree_builder != RTI_TREE_LBVH && p.tree_builder != RTI_TREE_PLOC)
     return fail(ctx, RTI_ERR_INVALID_ARG, "tree_builder must be 0, RTI_TREE_LBVH or RTI_TREE_PLOC");

@@ -111,8 +111,9 @@ struct trace_launch {
   int count;                        // 1 = instrumented run (node / prim / neighbour counters)
   int compact;                      // persistent: lanes done before a wave shades (1..64, 0 = default)
   int width;                        // traversal tree: 2 binary, 4 four-wide
-  int leaf_batch;                   // persistent four-wide: postponed leaf tests, a leaf phase once this many
-                                    // 64ths of the tracing lanes hold a queued leaf (1..64); 0 = at each node
+  int leaf_batch;                   // persistent four-wide: postponed leaf tests on the wave's leaf list, a leaf
+                                    // phase once the entries it takes reach this many 64ths of the lanes with
+                                    // a node left (1..256); 0 = at each node
   int refill_lanes;                 // persistent: lanes waiting for a new ray before a wave refills (1..64)
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);

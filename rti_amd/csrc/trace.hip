@@ -21,7 +21,8 @@
 //   binary: both child boxes in the parent (one 64-byte node per visit).
 // The per-lane stack keeps its top LS entries in LDS and the rest in private
 // memory. In the persistent kernel on the four-wide tree, hit leaves can be
-// queued per lane in LDS and tested in wave-wide leaf phases (leaf_batch).
+// put on a wave-shared list in LDS and tested in leaf phases, one entry per
+// lane whichever lane's ray it belongs to (leaf_batch).
 // Accumulation: per-primitive 64-bit integer atomics of round(v * 2^shift),
 // so totals are independent of the order of deposits (bit-exact run to run).
 
@@ -57,7 +58,7 @@ struct tparams {
   unsigned long long* queue;
   unsigned long long* stats;
   int compact;  // lanes without an open query before a wave shades and refills
-  int leaf_min;  // postponed leaves: 64ths of the tracing lanes that must hold one
+  int leaf_min;  // postponed leaves: listed leaves, in 64ths of the lanes with a node left, that start a leaf phase
   int refill_min;  // lanes waiting for a new ray before a wave refills (1..64)
   uint32_t block;  // threads per block (the LDS slot stride), uniform
 };
@@ -79,21 +80,21 @@ struct trav_state {
   int cur;          // node being visited (postponed leaves: -1 = no node left)
   float curd;       // entry distance of cur (postponed leaves: skipped beyond tbest)
   int sp;           // stack depth
-  uint32_t qa;      // LDS byte address of the next free leaf-queue slot (postponed leaves)
+  uint32_t lastq;   // postponed leaves: list sequence number after this lane's last entry
   uint32_t pbest;   // closest primitive so far
   bool found;
 };
 
-// Closest-hit update for one primitive (disc or triangle) in [tnear, tbest];
-// equal distances go to the lower primitive id.
+// Hit distance th >= tnear of the ray o + t d on primitive p (disc or
+// triangle); false when the ray misses it. No upper bound on th: the callers
+// compare it with their closest hit.
 template <bool COUNT>
-__device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3 d, uint32_t p, trav_state& T,
-                                          counters& C) {
+__device__ __forceinline__ bool prim_hit(const tparams& P, const f3 o, const f3 d, float tnear, uint32_t p,
+                                         float& th, counters& C) {
   if (COUNT) C.prims++;
   const float4* rec = P.prims + ((size_t)p << P.prim_shift);
   const float4 A = rec[0];
   const float4 B = rec[1];
-  float th;
   bool ok;
   if (A.w > 0.0f) {
     // oriented disc (Embree ORIENTED_DISC_POINT semantics: closest t in
@@ -103,7 +104,7 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
     const f3 co = make(A.x - o.x, A.y - o.y, A.z - o.z);
     th = dot(co, n) * __builtin_amdgcn_rcpf(dn);
     const f3 hv = make(fmaf(d.x, th, -co.x), fmaf(d.y, th, -co.y), fmaf(d.z, th, -co.z));
-    ok = (dn != 0.0f) && th >= T.tnear && th <= T.tbest && dot(hv, hv) < A.w * A.w;
+    ok = (dn != 0.0f) && th >= tnear && dot(hv, hv) < A.w * A.w;
   } else {
     // triangle: Moller-Trumbore
     const float4 C2 = rec[2];
@@ -116,9 +117,18 @@ __device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3
     const f3 qv = cross(tv, e1);
     const float v = dot(d, qv) * idet;
     th = dot(e2, qv) * idet;
-    ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= T.tnear && th <= T.tbest;
+    ok = (det != 0.0f) && u >= 0.0f && u <= 1.0f && v >= 0.0f && u + v <= 1.0f && th >= tnear;
   }
-  if (ok && (th < T.tbest || !T.found || p < T.pbest)) {
+  return ok;
+}
+
+// Closest-hit update for one primitive in [tnear, tbest]; equal distances go
+// to the lower primitive id.
+template <bool COUNT>
+__device__ __forceinline__ void test_prim(const tparams& P, const f3 o, const f3 d, uint32_t p, trav_state& T,
+                                          counters& C) {
+  float th;
+  if (prim_hit<COUNT>(P, o, d, T.tnear, p, th, C) && th <= T.tbest && (th < T.tbest || !T.found || p < T.pbest)) {
     T.tbest = th;
     T.pbest = p;
     T.found = true;
@@ -352,65 +362,116 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
   return true;
 }
 
-// Postponed leaf tests. A wave tests leaves only in leaf phases, where every
-// lane with a queued leaf tests one, instead of after each node visit, where
-// a node visit's leaves keep the whole wave while a few lanes test them.
-// Queued leaf j of thread t: the LDS word after the lane stacks at
-// j * block + t. A node visit queues at most four leaves. The queue holds 12
-// leaves beside a 4-entry lane stack and 8 beside deeper ones: 80 bytes per
-// thread with 4 or 6 stack entries, which leaves LDS for all 28 waves a CU's
-// registers allow, 96 with 8 (24 waves at 256 threads per block); 12 instead
-// of 8 entries measured +1.1 % on the hole (fewer phases forced by a full
-// queue), a 6-entry stack with 8 queue entries another +1.8 %.
-template <int LS>
-constexpr uint32_t leaf_queue_len() {
-  return LS <= 4 ? 12u : 8u;
-}
+// Postponed leaf tests. A wave tests leaves only in leaf phases instead of
+// after each node visit, where a node visit's leaves keep the whole wave
+// while a few lanes test them. Hit leaves go on a list shared by the wave's
+// lanes: a node phase appends every lane's hit leaves (an exclusive scan of
+// the per-lane counts, 0..4, from three ballots), a leaf phase takes the
+// oldest 64 entries and gives one to each lane, which fetches the owner's
+// ray (origin, direction, tnear) with ds_bpermute and tests the primitive.
+// The owners' closest hits are resolved by a 64-bit LDS atomic minimum on
+// keys (distance bits, primitive id): distances are >= 0, so their bits
+// order like the distances, and equal distances go to the lower id, the
+// order of test_prim. A lane's initial key is its closest hit so far, with
+// id 0xffffffff while it has none (a primitive at the boundary-face distance
+// then still wins), so the result does not depend on which lane tests which
+// entry, or when.
+//
+// LDS after the lane stacks: per wave kLeafList 64-bit entries (primitive id
+// in the low word, owner lane in the high word), a ring addressed from the
+// wave's head position; then one 64-bit key per thread. A node phase adds at
+// most four entries per lane with a node, and a leaf phase is forced before
+// one could overflow the ring.
+constexpr uint32_t kLeafList = 256;
 
 // Ray ids a persistent wave reserves beyond its lanes' needs.
 constexpr unsigned long long kRayReserve = 256;
 
-// The lane's queue is addressed by a running LDS byte address (T.qa) and
-// grows downwards, one slot (block * 4 bytes) per entry, from its empty top
-// at slot leaf_queue_len: queueing a leaf costs one subtract beside the write.
 template <int LS>
-__device__ __forceinline__ uint32_t leaf_queue_top(uint32_t stride) {
-  return (uint32_t)(size_t)((lds_u32*)(trace_lds) +
-                            (2u * LS * stride + leaf_queue_len<LS>() * stride + threadIdx.x));
+__device__ __forceinline__ uint32_t leaf_list_base(uint32_t stride) {
+  return (uint32_t)(size_t)((lds_u64*)(trace_lds) + (LS * stride + (threadIdx.x >> 6) * kLeafList));
 }
 
-__device__ __forceinline__ lds_u32* lds_word(uint32_t byte_addr) { return (lds_u32*)(size_t)byte_addr; }
+// Byte address of the wave's first key (lane 0's).
+template <int LS>
+__device__ __forceinline__ uint32_t hit_key_base(uint32_t stride) {
+  return (uint32_t)(size_t)((lds_u64*)(trace_lds) + (LS * stride + (stride >> 6) * kLeafList + (threadIdx.x & ~63u)));
+}
+
+__device__ __forceinline__ lds_u64* lds_dword(uint32_t byte_addr) { return (lds_u64*)(size_t)byte_addr; }
+
+__device__ __forceinline__ float lane_value(int src_byte, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
+}
+
+// Orders this wave's LDS accesses before the barrier ahead of those after it
+// (one wave's LDS instructions execute in issue order; the fences keep the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Closest-hit key of a distance (-0 taken as +0, so the bits order like
+// the value) and primitive id.
+__device__ __forceinline__ uint64_t hit_key(float t, uint32_t p) {
+  return ((uint64_t)__float_as_uint(t + 0.0f) << 32) | p;
+}
 
 // Node phase: visits T.cur unless its entry distance lies beyond the closest
-// hit found since it was chosen, queues its hit leaf children, descends into
-// the nearest hit internal child or pops; T.cur = -1 when no node is left.
+// hit found since it was chosen, descends into the nearest hit internal
+// child or pops (T.cur = -1 when no node is left). Returns the hit leaf
+// children as a slot mask, with the node's children in ch.
 template <int LS, bool COUNT>
-__device__ __forceinline__ void node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, counters& C) {
+__device__ __forceinline__ uint32_t node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, int4& ch,
+                                                counters& C) {
+  uint32_t leaves = 0;
   if (T.curd <= T.tbest) {
     node4_hits H;
     node4_boxes<COUNT>(P, T, H, C);
-    const uint32_t leaves = leaf_mask(H);
-    const uint32_t step = P.block * 4u;
-    uint32_t qa = T.qa;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (leaves & (1u << c)) {
-        qa -= step;
-        *lds_word(qa) = sel4(H.ch, (uint32_t)c);  // ~prim
-      }
-    }
-    T.qa = qa;
-    if (order_push<LS, false>(P, T, H, ovf)) return;
+    leaves = leaf_mask(H);
+    ch = H.ch;
+    if (order_push<LS, false>(P, T, H, ovf)) return leaves;
   }
   if (!pop4<LS>(P, T, ovf)) T.cur = -1;
+  return leaves;
 }
 
-// Leaf phase: the most recently queued leaf of this lane.
-template <int LS, bool COUNT>
-__device__ __forceinline__ void leaf_visit(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
-  const uint32_t p = ~*lds_word(T.qa);
-  T.qa += P.block * 4u;
-  test_prim<COUNT>(P, o, d, p, T, C);
+// Leaf phase over the list entries [head, head + take) (ring positions),
+// take <= 64; runs with every lane of the wave active (ds_bpermute reads
+// the owners' registers).
+template <bool COUNT>
+__device__ __forceinline__ void leaf_phase(const tparams& P, const f3 ro, const f3 rd, trav_state& T, uint32_t list,
+                                           uint32_t keys, uint32_t head, uint32_t take, uint32_t lane,
+                                           counters& C) {
+  lds_u64* mine = lds_dword(keys + lane * 8u);
+  *mine = hit_key(T.tbest, T.found ? T.pbest : 0xffffffffu);
+  wave_lds_sync();
+  const bool tester = lane < take;
+  uint64_t e = 0;
+  if (tester) {
+    uint32_t pos = head + lane;
+    if (pos >= kLeafList) pos -= kLeafList;
+    e = *lds_dword(list + pos * 8u);
+  }
+  const uint32_t owner = (uint32_t)(e >> 32);
+  const int src = (int)(owner << 2);
+  const f3 o = make(lane_value(src, ro.x), lane_value(src, ro.y), lane_value(src, ro.z));
+  const f3 d = make(lane_value(src, rd.x), lane_value(src, rd.y), lane_value(src, rd.z));
+  const float tn = lane_value(src, T.tnear);
+  if (tester) {
+    const uint32_t p = (uint32_t)e;
+    float th;
+    if (prim_hit<COUNT>(P, o, d, tn, p, th, C))
+      __hip_atomic_fetch_min(lds_dword(keys + owner * 8u), hit_key(th, p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+  wave_lds_sync();
+  const uint64_t k = *mine;
+  T.tbest = __uint_as_float((uint32_t)(k >> 32));
+  T.pbest = (uint32_t)k;
+  T.found = T.pbest != 0xffffffffu;
 }
 
 // Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
@@ -435,8 +496,8 @@ struct lane_stack<LS, 4> {
     return trav4_step<LS, COUNT>(P, o, d, T, ovf, C);
   }
   template <bool COUNT>
-  __device__ __forceinline__ void visit(const tparams& P, trav_state& T, counters& C) {
-    node4_visit<LS, COUNT>(P, T, ovf, C);
+  __device__ __forceinline__ uint32_t visit(const tparams& P, trav_state& T, int4& ch, counters& C) {
+    return node4_visit<LS, COUNT>(P, T, ovf, ch, C);
   }
 };
 
@@ -621,10 +682,11 @@ __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_st
 
 // Start of a segment: the boundary-face distance bounds the closest-hit
 // query, which starts at the instance's root.
-// qtop: the lane's empty leaf-queue address (postponed leaves; unused otherwise).
+// qseq: the wave's leaf-list head sequence number (postponed leaves; no
+// entry of this lane is pending).
 template <int W>
 __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C,
-                                              uint32_t qtop) {
+                                              uint32_t qseq) {
   const inst_desc& D = P.desc[R.inst];
   C.seg++;
   R.segs++;
@@ -645,7 +707,7 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.cur = W == 4 ? D.root4 : D.root;
   T.curd = 0.0f;
   T.sp = 0;
-  T.qa = qtop;
+  T.lastq = qseq;
   T.oi = make(R.o.x * T.inv.x, R.o.y * T.inv.y, R.o.z * T.inv.z);
 }
 
@@ -781,12 +843,12 @@ __device__ __forceinline__ void flush_counters(const tparams& P, const counters&
 // (waves_per_cu) where the memory system prefers it.
 //
 // With postponed leaves (PL, four-wide tree) each traversal iteration is
-// either a node phase (lanes with a node left visit it and queue its hit
-// leaves) or a leaf phase (lanes with a queued leaf test one), chosen for
-// the whole wave: a leaf phase runs once at least leaf_min / 64 of the
-// tracing lanes hold a queued leaf, when a lane's queue could not take
-// another node's four, or when no node is left. A lane's query is open
-// while it has a node or a queued leaf left.
+// either a node phase (lanes with a node left visit it and append its hit
+// leaves to the wave's list) or a leaf phase (up to 64 list entries tested,
+// one per lane), chosen for the whole wave: a leaf phase runs once the
+// entries it would take reach leaf_min / 64 of the lanes with a node left,
+// when the list could not take another node phase's leaves, or when no node
+// is left. A lane's query is open while it has a node or a listed leaf left.
 template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT, bool PL>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? 6 : 7))) void k_trace_persistent(
     tparams P) {
@@ -806,10 +868,11 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   // does not change any result.
   unsigned long long res_base = 0, res_end = 0;  // reserved ids [res_base, res_end), wave-uniform
   const unsigned long long tail = (unsigned long long)gridDim.x * (P.block >> 6) * 64ull * 16ull;
-  // the lane's leaf queue: empty at qtop, no room for another node's four
-  // leaves below qfull
-  const uint32_t qtop = PL ? leaf_queue_top<LS>(P.block) : 0u;
-  const uint32_t qfull = qtop - (leaf_queue_len<LS>() - 4) * P.block * 4u;
+  // the wave's leaf list (postponed leaves): wave-uniform head position in
+  // the ring, head sequence number and entry count
+  const uint32_t list = PL ? leaf_list_base<LS>(P.block) : 0u;
+  const uint32_t keys = PL ? hit_key_base<LS>(P.block) : 0u;
+  uint32_t lhead = 0, lseq = 0, lcount = 0;
   for (;;) {
     // ended rays are replaced once refill_min lanes wait for one (a refill
     // costs the wave the same whatever the number of lanes it serves), or
@@ -837,7 +900,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
       if (!alive && !exhausted) {
         if (my < P.ray_hi) {
           start_ray(P, my, R);
-          begin_segment<W>(P, R, T, C, qtop);
+          begin_segment<W>(P, R, T, C, lseq);
           alive = true;
           tracing = true;
         } else {
@@ -860,19 +923,46 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
       // flag into a 0/1 register and compare that.
       unsigned long long tr = __ballot(tracing);
       for (;;) {
-        const unsigned long long queued = tr & __ballot(T.qa != qtop);
-        const unsigned long long full = tr & __ballot(T.qa < qfull);
-        const unsigned long long nodes = tr & __ballot(T.cur >= 0);
-        const bool leaf_phase =
-            queued != 0ull && (full != 0ull || nodes == 0ull || __popcll(queued) * 64 >= P.leaf_min * __popcll(tr));
-        if (leaf_phase) {
-          if (tracing && T.qa != qtop) leaf_visit<LS, COUNT>(P, R.o, R.d, T, C);
+        const uint32_t nl = (uint32_t)__popcll(tr & __ballot(T.cur >= 0));
+        const uint32_t take = lcount < 64u ? lcount : 64u;
+        const bool leaf_phase_now =
+            lcount != 0u && (nl == 0u || lcount + 4u * nl > kLeafList || take * 64u >= (uint32_t)P.leaf_min * nl);
+        if (leaf_phase_now) {
+          leaf_phase<COUNT>(P, R.o, R.d, T, list, keys, lhead, take, (uint32_t)lane, C);
+          lhead += take;
+          if (lhead >= kLeafList) lhead -= kLeafList;
+          lseq += take;
+          lcount -= take;
         } else {
-          if (tracing && T.cur >= 0) S.template visit<COUNT>(P, T, C);
+          uint32_t lm = 0;
+          int4 ch = make_int4(0, 0, 0, 0);
+          if (tracing && T.cur >= 0) lm = S.template visit<COUNT>(P, T, ch, C);
+          // append the hit leaves: exclusive scan of the counts (0..4)
+          const uint32_t n = (uint32_t)__popc(lm);
+          const unsigned long long b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
+          const uint32_t before =
+              (uint32_t)(__popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt));
+          const uint32_t added = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+          if (lm) {
+            uint32_t pos = lhead + lcount;  // the list's end: lhead, lcount < kLeafList here
+            if (pos >= kLeafList) pos -= kLeafList;
+            pos += before;
+            const uint64_t own = (uint64_t)lane << 32;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              if (lm & (1u << c)) {
+                if (pos >= kLeafList) pos -= kLeafList;
+                *lds_dword(list + pos * 8u) = own | (uint32_t)(~sel4(ch, (uint32_t)c));
+                ++pos;
+              }
+            }
+            T.lastq = lseq + lcount + before + n;
+          }
+          lcount += added;
         }
-        const bool more = T.cur >= 0 || T.qa != qtop;
-        tr &= __ballot(T.cur >= 0) | __ballot(T.qa != qtop);
-        tracing = tracing && more;
+        const bool pend = (int32_t)(T.lastq - lseq) > 0;
+        tr &= __ballot(T.cur >= 0) | __ballot(pend);
+        tracing = tracing && (T.cur >= 0 || pend);
         if (__popcll(tr) <= still) break;
       }
     } else {
@@ -889,7 +979,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     if (alive && !tracing) {
       alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
       if (alive) {
-        begin_segment<W>(P, R, T, C, qtop);
+        begin_segment<W>(P, R, T, C, lseq);
         tracing = true;
       }
     }
@@ -958,11 +1048,11 @@ kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool a
   return nullptr;
 }
 
-// Dynamic LDS: the lane stacks, then (postponed leaves) the leaf queues.
+// Dynamic LDS: the lane stacks, then (postponed leaves) the waves' leaf
+// lists and the threads' closest-hit keys.
 size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) {
   size_t b = (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
-  if (width == 4 && postpone)
-    b += (size_t)block_size * (lds_stack <= 4 ? leaf_queue_len<4>() : leaf_queue_len<8>()) * 4;
+  if (width == 4 && postpone) b += (size_t)(block_size / 64) * kLeafList * 8 + (size_t)block_size * 8;
   return b;
 }
 
@@ -973,9 +1063,10 @@ size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) 
 int default_lds_stack(int width, bool postpone) { return width == 4 ? (postpone ? 6 : 8) : 16; }
 int default_compact_lanes() { return 56; }
 // Postponed leaves pay off where a leaf test is expensive: triangle stores
-// (measured +5 % on a 199k-triangle hole) but not stores of discs only, whose
-// leaf test is a few instructions (batch of disc clouds -5 %, rough surface -3 %).
-int default_leaf_batch(bool triangles) { return triangles ? 48 : 0; }
+// (measured +27 % on a 199k-triangle hole, flat within 1 % for thresholds
+// 64..192) but not stores of discs only, whose leaf test is a few
+// instructions (batch of disc clouds -2 %, rough surface -2 % at 64).
+int default_leaf_batch(bool triangles) { return triangles ? 128 : 0; }
 int wide_stack_capacity() { return kWideStack; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
@@ -995,7 +1086,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   P.queue = L.queue;
   P.stats = L.stats;
   P.compact = L.compact > 0 ? (L.compact > 64 ? 64 : L.compact) : default_compact_lanes();
-  P.leaf_min = L.leaf_batch > 64 ? 64 : L.leaf_batch;
+  P.leaf_min = L.leaf_batch > 256 ? 256 : L.leaf_batch;
   P.refill_min = L.refill_lanes < 1 ? 1 : (L.refill_lanes > 64 ? 64 : L.refill_lanes);
   const bool persistent = L.kernel == 0;
   const bool postpone = persistent && L.width == 4 && L.leaf_batch > 0;

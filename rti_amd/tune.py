@@ -7,8 +7,9 @@ closest-hit query before the wave shades and refills), then the resident
 waves per CU (0 = as many as registers and LDS allow; fewer can suit
 workloads whose working set outgrows the caches), then the LDS stack depth,
 then the block size, then ``leaf_batch`` (-1: leaf tests at each node visit;
-32, 48, 64: postponed leaf tests in wave-wide leaf phases, started once that
-many 64ths of the tracing lanes hold a queued leaf), then ``refill_lanes``
+32..256: postponed leaf tests on the wave's shared leaf list, a leaf phase
+once the entries it takes reach that many 64ths of the lanes with a node
+left), then ``refill_lanes``
 (lanes of a wave that must wait for a new ray before the wave refills). Results are cached in a JSON file keyed by the
 batch signature, so a repeated workload is not re-tuned.
 """
@@ -23,7 +24,7 @@ from ._lib import Context, device_info
 COMPACT = (40, 48, 52, 56, 60, 64)
 LDS = (4, 6, 8, 16)
 BLOCK = (128, 256)
-LEAF = (-1, 32, 48, 64)
+LEAF = (-1, 64, 128, 256)
 REFILL = (1, 8, 16)
 WAVES = (0, 20, 24)
 MIN_GAIN = 0.005  # relative gain a coordinate step must show to change a knob

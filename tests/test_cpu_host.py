@@ -291,7 +291,7 @@ def test_tuner_descent_ignores_gains_within_noise(monkeypatch):
             r *= 1.001
         if knobs["lds_stack"] == 4 and knobs["waves_per_cu"] == 0:
             r *= 1.018
-        if knobs["leaf_batch"] == 48:
+        if knobs["leaf_batch"] == 128:
             r *= 1.01
         return r
 
@@ -302,7 +302,7 @@ def test_tuner_descent_ignores_gains_within_noise(monkeypatch):
     k = tune.knobs(out)
     assert k["waves_per_cu"] == 0
     assert k["lds_stack"] == 4
-    assert k["leaf_batch"] == 48
+    assert k["leaf_batch"] == 128
     # the reported rate is that of the knobs returned
     assert out["segments_per_s"] == rate(None, k)
     # every knob was visited: six knobs, block size and leaf threshold twice

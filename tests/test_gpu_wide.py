@@ -156,20 +156,32 @@ def test_wide_and_binary_traversal_agree_bit_for_bit(which, kw):
     b = run_scene(sc, bvh_width=2, **base)
     b6 = run_scene(sc, bvh_width=2, lds_stack=6, block_size=192, **base)
     c = run_scene(sc, bvh_width=4, kernel=1, **base)
-    # leaf tests at each node visit, and postponed leaf tests with a leaf
-    # phase as soon as one lane queues a leaf / only when every tracing lane
-    # holds one (small block and LDS stack: the private stack part is used)
+    # leaf tests at each node visit, and postponed leaf tests on the wave's
+    # leaf list with a leaf phase as soon as one leaf is listed / only once
+    # the listed leaves reach four per lane with a node (small block and LDS
+    # stack: the private stack part is used)
     d = run_scene(sc, bvh_width=4, leaf_batch=-1, **base)
     e = run_scene(sc, bvh_width=4, leaf_batch=1, **base)
-    f = run_scene(sc, bvh_width=4, leaf_batch=64, block_size=64, lds_stack=4, **base)
+    f = run_scene(sc, bvh_width=4, leaf_batch=256, block_size=64, lds_stack=4, **base)
     f6 = run_scene(sc, bvh_width=4, leaf_batch=48, block_size=128, lds_stack=6, **base)
     # the other binary-tree topology: the closest hit does not depend on the tree
     g = run_scene(sc, tree_builder=1, **base)
     h = run_scene(sc, tree_builder=2, **base)
-    for other in (b, b6, c, d, e, f, f6, g, h):
+    # the counting kernel on the wave-shared leaf list, and with tnear = 0
+    # (hit distances of -0 and +0 must order alike)
+    k = run_scene(sc, bvh_width=4, leaf_batch=32, kernel=2, **base)
+    for other in (b, b6, c, d, e, f, f6, g, h, k):
         assert np.array_equal(a["counts"], other["counts"])
         assert np.array_equal(a["raw"], other["raw"])
         assert a["stats"]["segments"] == other["stats"]["segments"]
+    z0 = run_scene(sc, bvh_width=4, leaf_batch=-1, tnear=0.0, **base)
+    z1 = run_scene(sc, bvh_width=4, leaf_batch=16, tnear=0.0, **base)
+    assert np.array_equal(z0["raw"], z1["raw"]) and np.array_equal(z0["counts"], z1["counts"])
+    # adjoint sums with and without postponed leaves
+    j0 = run_scene(sc, bvh_width=4, leaf_batch=-1, adjoint=1, **base)
+    j1 = run_scene(sc, bvh_width=4, leaf_batch=48, adjoint=1, **base)
+    for key in ("raw", "counts", "ds", "dn"):
+        assert np.array_equal(j0[key], j1[key]), key
 
 
 def test_wide_tree_visits_fewer_nodes():
@@ -190,7 +202,7 @@ def test_invalid_width_is_rejected():
 
 def test_invalid_leaf_batch_is_rejected():
     with Context() as ctx:
-        for v in (-2, 65):
+        for v in (-2, 257):
             with pytest.raises(Exception):
                 ctx.set_params(leaf_batch=v)
         for v in (-1, 65):
