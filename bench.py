@@ -278,8 +278,10 @@ def profile_figure(name, key):
     one. Keys: "bytes_per_segment" (HBM-side traffic from FETCH_SIZE and
     WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
     known size), "l2_hit_rate" (TCC_HIT / (TCC_HIT + TCC_MISS)), "valu_busy"
-    (SQ_ACTIVE_INST_VALU over the SIMDs' busy cycles), "active_lanes"
-    (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64)."""
+    (rocprof's VALUBusy: SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE of
+    one XCC; above 1 when two VALU instructions issue in one quad-cycle),
+    "valu_dual_issue" (SQ_ACTIVE_INST_VALU2 / SQ_ACTIVE_INST_VALU),
+    "active_lanes" (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64)."""
     path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
     if not os.path.exists(path):
         return None
@@ -384,13 +386,20 @@ def main():
     traffic = None if b_meas is None else b_meas * segments
     traffic_gbs = None if traffic is None else traffic / (ms_per_step / 1e3) / 1e9
     valu_busy = profile_figure(args.workload, "valu_busy")
+    dual = profile_figure(args.workload, "valu_dual_issue")
     lanes = profile_figure(args.workload, "active_lanes")
     # the resource that binds, from the committed counters of this workload:
-    # VALU issue when the SIMDs are nearly always issuing while HBM-side
-    # traffic stays well below the peak, HBM otherwise
+    # HBM when the traffic reaches 0.7 of the peak; VALU issue when the SIMDs
+    # are nearly always issuing VALU work (VALUBusy >= 0.85) below that;
+    # otherwise neither is saturated: latency and the launch's tail
     binding = None
     if valu_busy is not None and traffic_gbs is not None:
-        binding = "valu" if valu_busy >= 0.85 and traffic_gbs < 0.7 * HBM_PEAK_GBS else "hbm"
+        if traffic_gbs >= 0.7 * HBM_PEAK_GBS:
+            binding = "hbm"
+        elif valu_busy >= 0.85:
+            binding = "valu"
+        else:
+            binding = "latency"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_gbs": None if traffic_gbs is None else round(traffic_gbs, 1),
@@ -398,14 +407,15 @@ def main():
                 "l2_hit_rate": profile_figure(args.workload, "l2_hit_rate"),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
                 "bytes_per_segment_records": round(B_rec, 1),
-                "binding": binding, "valu_busy": valu_busy, "active_lanes": lanes,
-                "useful_valu_frac": None if (valu_busy is None or lanes is None) else round(valu_busy * lanes / 64, 3),
+                "binding": binding, "valu_busy": valu_busy, "valu_dual_issue": dual, "active_lanes": lanes,
+                "lane_frac": None if lanes is None else round(lanes / 64, 3),
                 "note": "achieved and frac: B_seg = 32 V_node + 32 V_prim + 24 D + 36 K_nb + 64 bytes per segment "
                         "(V_node four-wide node visits) x segments/s over the HBM peak; bytes_per_segment_records "
                         "prices the same visits at this layout's record sizes. traffic, traffic_gbs, l2_hit_rate, "
-                        "valu_busy (VALU issue over busy SIMD cycles) and active_lanes (of 64, per issued VALU "
-                        "instruction) come from the committed counter summary of the trace kernel on this "
-                        "workload (null without one); binding names the resource those counters show binding"}
+                        "valu_busy (rocprof VALUBusy; above 1 with dual VALU issue), valu_dual_issue and "
+                        "active_lanes (of 64, per issued VALU instruction) come from the committed counter summary "
+                        "of the trace kernel on this workload (null without one); binding names the resource "
+                        "those counters show binding (hbm, valu, or latency when neither is saturated)"}
 
     cpu = None
     if not args.no_cpu:
