@@ -430,6 +430,18 @@ int prepare(rti_ctx* ctx) {
     J.node4_begin = I.node4_begin;
     centroid_bounds(I, J.cmin, J.cmax);
     J.rmax = I.rmax;
+    // Leaf boxes are padded by 2e-6 of the instance's coordinate scale (its
+    // extent and source plane), four times the rounding error of a slab
+    // distance against that of a primitive's own hit distance for ray origins
+    // of that scale: a primitive hit at exactly the closest distance found so
+    // far (coplanar discs tie exactly) then always lies inside the boxes the
+    // traversal keeps, whatever the order of the tests.
+    {
+      float m = std::max(1.0f, std::fabs(I.ztop));
+      for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(I.geo_lo[k]), std::fabs(I.geo_hi[k])));
+      if (!std::isnan(I.params.src_z)) m = std::max(m, std::fabs(I.params.src_z));
+      J.pad = 2e-6f * m;
+    }
     J.area_box[0] = I.box[0];
     J.area_box[1] = I.box[2];
     J.area_box[2] = I.box[1];

@@ -173,6 +173,7 @@ struct setup_args {
   const float* verts;
   const uint32_t* tris;
   float cmin0, cmin1, cmin2, cinv0, cinv1, cinv2;
+  float pad;  // absolute padding of every leaf box
 };
 
 __device__ __forceinline__ void prim_centroid(const setup_args& a, uint32_t i, float& x, float& y, float& z) {
@@ -232,7 +233,7 @@ __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t pr
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       // oriented disc extent along axis k: r * sqrt(1 - n_k^2), padded
-      float ext = c.w * sqrtf(fmaxf(0.0f, 1.0f - nn[k] * nn[k])) + c.w * 1e-5f;
+      float ext = c.w * sqrtf(fmaxf(0.0f, 1.0f - nn[k] * nn[k])) + c.w * 1e-5f + a.pad;
       lo[k] = cc[k] - ext;
       hi[k] = cc[k] + ext;
     }
@@ -259,7 +260,7 @@ __global__ void k_scatter_prims(setup_args a, const uint32_t* order, uint32_t pr
     for (int k = 0; k < 3; ++k) {
       float l = fminf(v[0][k], fminf(v[1][k], v[2][k]));
       float h = fmaxf(v[0][k], fmaxf(v[1][k], v[2][k]));
-      float pad = 1e-6f * fmaxf(1.0f, fmaxf(fabsf(l), fabsf(h)));
+      float pad = fmaxf(a.pad, 1e-6f * fmaxf(1.0f, fmaxf(fabsf(l), fabsf(h))));
       lo[k] = l - pad;
       hi[k] = h + pad;
     }
@@ -972,6 +973,7 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
   for (int k = 0; k < 3; ++k) inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
   a.cmin0 = job.cmin[0]; a.cmin1 = job.cmin[1]; a.cmin2 = job.cmin[2];
   a.cinv0 = inv[0]; a.cinv1 = inv[1]; a.cinv2 = inv[2];
+  a.pad = job.pad;
   const unsigned T = 256;
   k_morton<<<blocks_for(n, T), T, 0, stream>>>(a, s.keys[0], s.vals[0]);
   // 30-bit keys: four 8-bit passes, result back in buffer 0

@@ -24,6 +24,7 @@ struct build_job {
   const uint32_t* d_tris;   // triangles, n x 3
   float cmin[3], cmax[3];   // centroid bounds (for Morton quantisation)
   float rmax;               // largest disc radius (neighbour query reach)
+  float pad;                // absolute padding of every primitive box (coordinate scale of the instance)
   float area_box[4];        // xmin, ymin, xmax, ymax for the exposed area
   int builder;              // binary tree topology: kBuildKarras or kBuildPloc
 };

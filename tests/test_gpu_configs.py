@@ -106,6 +106,31 @@ def test_c3_batch_at_own_budgets():
              f"s={c['sticking']:.3g}, diffuse {c['diffuse_prob']}): {z.size} elements, z stats {check_z(z)}")
 
 
+def test_c3_batch_independent_of_leaf_scheduling_and_tree():
+    """Coplanar discs tie at exactly equal distances; the lower stored
+    position wins only if culling never drops a tied partner. At the batch's
+    full size (6.4e8 rays, 8.4e9 segments) every instance is bit-identical
+    across leaf tests at each node visit, the wave-shared leaf list under two
+    schedules, and the binary tree."""
+    _, cases = _batch_cases()
+    _say("c3 batch: leaf scheduling and tree width, 6.4e8 rays each")
+    runs = []
+    for knobs in (dict(leaf_batch=-1), dict(leaf_batch=64, compact_lanes=56, refill_lanes=1),
+                  dict(leaf_batch=64, compact_lanes=52, refill_lanes=8), dict(bvh_width=2)):
+        with Context() as ctx:
+            ctx.batch_set(cases)
+            ctx.set_params(**knobs)
+            ctx.batch_run()
+            runs.append(([ctx.batch_flux(i)["raw"] for i in range(64)], [ctx.batch_counts(i) for i in range(64)],
+                         int(ctx.counters()[0])))
+        _say(f"c3 {knobs}: {runs[-1][2]} segments")
+    for r in runs[1:]:
+        assert r[2] == runs[0][2]
+        for k in range(64):
+            assert np.array_equal(r[0][k], runs[0][0][k]), k
+            assert np.array_equal(r[1][k], runs[0][1][k]), k
+
+
 def test_c4_adjoint_vs_crn_finite_differences():
     """Summed d(flux)/ds and d(flux)/dn over the whole trench: adjoint mean
     over 16 seeds against the CRN central differences' mean. Bound: 2 % of
