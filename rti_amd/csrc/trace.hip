@@ -400,6 +400,12 @@ __device__ __forceinline__ uint32_t hit_key_base(uint32_t stride) {
 
 __device__ __forceinline__ lds_u64* lds_dword(uint32_t byte_addr) { return (lds_u64*)(size_t)byte_addr; }
 
+// Set bits of a wave mask in the lanes below this one (v_mbcnt: two
+// instructions, against an and and a popcount per half).
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 __device__ __forceinline__ float lane_value(int src_byte, float v) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
 }
@@ -858,7 +864,6 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   trav_state T;
   bool alive = false, exhausted = false, tracing = false;
   const int lane = threadIdx.x & 63;
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int compact = P.compact;
   // Ray ids come from the global queue in blocks: a wave reserves
   // kRayReserve ids beyond what its lanes need, so most refills hand out ids
@@ -880,7 +885,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     const unsigned long long need = __ballot(!alive && !exhausted);
     if (need && (__popcll(need) >= P.refill_min || __ballot(alive) == 0ull)) {
       const unsigned long long cnt = (unsigned long long)__popcll(need);
-      const unsigned long long rank = (unsigned long long)__popcll(need & lt);
+      const unsigned long long rank = lanes_below(need);
       const unsigned long long have = res_end - res_base;
       unsigned long long my;
       if (cnt <= have) {
@@ -940,23 +945,23 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
           // append the hit leaves: exclusive scan of the counts (0..4)
           const uint32_t n = (uint32_t)__popc(lm);
           const unsigned long long b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
-          const uint32_t before =
-              (uint32_t)(__popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt));
+          const uint32_t before = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
           const uint32_t added = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
           if (lm) {
             uint32_t pos = lhead + lcount;  // the list's end: lhead, lcount < kLeafList here
             if (pos >= kLeafList) pos -= kLeafList;
             pos += before;
             const uint64_t own = (uint64_t)lane << 32;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              if (lm & (1u << c)) {
-                if (pos >= kLeafList) pos -= kLeafList;
-                *lds_dword(list + pos * 8u) = own | (uint32_t)(~sel4(ch, (uint32_t)c));
-                ++pos;
-              }
-            }
             T.lastq = lseq + lcount + before + n;
+            // one trip per hit leaf of the busiest lane (mostly one), not four
+            // predicated slots
+            do {
+              const uint32_t c = (uint32_t)(__ffs(lm) - 1);
+              lm &= lm - 1u;
+              if (pos >= kLeafList) pos -= kLeafList;
+              *lds_dword(list + pos * 8u) = own | (uint32_t)(~sel4(ch, c));
+              ++pos;
+            } while (lm);
           }
           lcount += added;
         }
