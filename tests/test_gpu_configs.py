@@ -1,5 +1,6 @@
-"""GPU tests at the shapes of the benchmark configurations that are not the
-default bench workload:
+"""GPU tests at the shapes of the benchmark configurations: the default
+bench workload's hole at full size traced under different schedules and
+trees, and the configurations that are not the default bench workload:
 
 * the 64-instance disc batch (trenches and holes of aspect 2-20, 1e4-5e5
   discs, sticking 1e-3..1, diffuse or mirror) at its own ray budgets (6.4e8
@@ -127,6 +128,30 @@ def test_c3_batch_independent_of_leaf_scheduling_and_tree():
     for r in runs[1:]:
         assert r[2] == runs[0][2]
         for k in range(64):
+            assert np.array_equal(r[0][k], runs[0][0][k]), k
+            assert np.array_equal(r[1][k], runs[0][1][k]), k
+
+
+def test_c2_hole_independent_of_leaf_scheduling_and_tree():
+    """The benchmark's hole at its full size (six (s, n) cases of 1e8 rays on
+    one shared 199,360-triangle mesh): bit-identical sums with the leaf list
+    at two thresholds, with leaf tests at each node visit, and on the binary
+    tree."""
+    sc = geometry.hole_triangles()
+    cases = [dict(scene=sc, n_rays=100_000_000, sticking=s, cos_exponent=n, bc_x=1, bc_y=1, seed=2)
+             for s in (0.01, 0.1, 1.0) for n in (1.0, 10.0)]
+    runs = []
+    for knobs in (dict(), dict(leaf_batch=1, compact_lanes=48), dict(leaf_batch=-1), dict(bvh_width=2)):
+        with Context() as ctx:
+            ctx.batch_set(cases)
+            ctx.set_params(**knobs)
+            ctx.batch_run()
+            runs.append(([ctx.batch_flux(i)["raw"] for i in range(6)], [ctx.batch_counts(i) for i in range(6)],
+                         int(ctx.counters()[0])))
+        _say(f"c2 hole {knobs}: {runs[-1][2]} segments")
+    for r in runs[1:]:
+        assert r[2] == runs[0][2]
+        for k in range(6):
             assert np.array_equal(r[0][k], runs[0][0][k]), k
             assert np.array_equal(r[1][k], runs[0][1][k]), k
 
