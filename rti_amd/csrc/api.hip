@@ -622,6 +622,13 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
   }
   cfg.leaf = p.leaf_batch < 0 ? 0
              : (p.leaf_batch ? p.leaf_batch : rti_impl::default_leaf_batch(ctx->prim_shift == rti_dev::kPrimShiftMixed));
+  if (cfg.width == 4 && cfg.leaf > 0 && ctx->n_total >= rti_impl::postponed_leaf_prim_limit()) {
+    // leaf words on the wave's list hold 25-bit primitive ids
+    if (p.leaf_batch > 0)
+      return fail(ctx, RTI_ERR_INVALID_ARG, "leaf_batch > 0 needs fewer than %llu primitives in the store (%llu)",
+                  (unsigned long long)rti_impl::postponed_leaf_prim_limit(), (unsigned long long)ctx->n_total);
+    cfg.leaf = 0;
+  }
   cfg.lds = p.lds_stack ? p.lds_stack : rti_impl::default_lds_stack(cfg.width, cfg.width == 4 && cfg.leaf > 0);
   if (cfg.width == 4) {
     for (auto& I : ctx->inst)

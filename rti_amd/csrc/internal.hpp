@@ -125,6 +125,8 @@ int default_lds_stack(int width, bool postpone);
 int wide_stack_capacity();
 int default_compact_lanes();
 int default_leaf_batch(bool triangles);  // 0 = leaf tests at each node
+// Postponed leaf tests list primitive ids below this bound.
+uint64_t postponed_leaf_prim_limit();
 
 // ---- post ----
 hipError_t fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);

@@ -365,40 +365,55 @@ __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f
 // Postponed leaf tests. A wave tests leaves only in leaf phases instead of
 // after each node visit, where a node visit's leaves keep the whole wave
 // while a few lanes test them. Hit leaves go on a list shared by the wave's
-// lanes: a node phase appends every lane's hit leaves (an exclusive scan of
-// the per-lane counts, 0..4, from three ballots), a leaf phase takes the
-// oldest 64 entries and gives one to each lane, which fetches the owner's
-// ray (origin, direction, tnear) with ds_bpermute and tests the primitive.
-// The owners' closest hits are resolved by a 64-bit LDS atomic minimum on
-// keys (distance bits, primitive id): distances are >= 0, so their bits
-// order like the distances, and equal distances go to the lower id, the
-// order of test_prim. A lane's initial key is its closest hit so far, with
-// id 0xffffffff while it has none (a primitive at the boundary-face distance
-// then still wins), so the result does not depend on which lane tests which
-// entry, or when.
+// lanes, one 16-byte group per visited node with hit leaves: the node's four
+// child words, a hit leaf child as a leaf word (owner lane and primitive id),
+// any other slot 0. A node phase appends the groups with one 16-byte write
+// per lane (a ballot ranks the writing lanes); a leaf phase takes the oldest
+// groups whose leaves fit 64 lanes, compacts their leaf words in place (each
+// group's lane writes its leaves at the exclusive scan of the groups' leaf
+// counts) and gives one leaf word to each lane, whichever lane's ray it
+// belongs to: the tester fetches the owner's origin, direction and tnear with
+// ds_bpermute, tests the primitive, and resolves the owner's closest hit by a
+// 64-bit LDS atomic minimum on keys (distance bits, primitive id): distances
+// are >= 0, so their bits order like the distances, and equal distances go to
+// the lower id, the order of test_prim. A lane's initial key is its closest
+// hit so far, with id 0xffffffff while it has none (a primitive at the
+// boundary-face distance then still wins), so the result does not depend on
+// which lane tests which leaf, or when.
 //
-// LDS after the lane stacks: per wave kLeafList 64-bit entries (primitive id
-// in the low word, owner lane in the high word), a ring addressed from the
-// wave's head position; then one 64-bit key per thread. A node phase adds at
-// most four entries per lane with a node, and a leaf phase is forced before
-// one could overflow the ring.
-constexpr uint32_t kLeafList = 256;
+// LDS after the lane stacks: per wave a ring of kLeafGroups groups (512
+// words), addressed from the wave's head group; then one 64-bit key per
+// thread. A node phase adds at most one group per lane with a node, and a
+// leaf phase is forced before one could overflow the ring.
+constexpr uint32_t kLeafGroups = 128;
+constexpr uint32_t kLeafWords = 4 * kLeafGroups;
+// Leaf word: bit 31 set, owner lane in bits 25..30, primitive id below (so
+// postponed leaves need primitive ids < 2^25; the host checks).
+constexpr uint32_t kLeafPrimBits = 25;
+constexpr uint32_t kLeafPrimMask = (1u << kLeafPrimBits) - 1u;
 
 // Ray ids a persistent wave reserves beyond its lanes' needs.
 constexpr unsigned long long kRayReserve = 256;
 
+// Byte address of the wave's leaf ring (16 bytes per group).
 template <int LS>
 __device__ __forceinline__ uint32_t leaf_list_base(uint32_t stride) {
-  return (uint32_t)(size_t)((lds_u64*)(trace_lds) + (LS * stride + (threadIdx.x >> 6) * kLeafList));
+  return (uint32_t)(size_t)((lds_u64*)(trace_lds) + (LS * stride + (threadIdx.x >> 6) * (kLeafGroups * 2)));
 }
 
 // Byte address of the wave's first key (lane 0's).
 template <int LS>
 __device__ __forceinline__ uint32_t hit_key_base(uint32_t stride) {
-  return (uint32_t)(size_t)((lds_u64*)(trace_lds) + (LS * stride + (stride >> 6) * kLeafList + (threadIdx.x & ~63u)));
+  return (uint32_t)(size_t)((lds_u64*)(trace_lds) +
+                            (LS * stride + (stride >> 6) * (kLeafGroups * 2) + (threadIdx.x & ~63u)));
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u128;
+
 __device__ __forceinline__ lds_u64* lds_dword(uint32_t byte_addr) { return (lds_u64*)(size_t)byte_addr; }
+__device__ __forceinline__ lds_u32* lds_word(uint32_t byte_addr) { return (lds_u32*)(size_t)byte_addr; }
+__device__ __forceinline__ lds_u128* lds_quad(uint32_t byte_addr) { return (lds_u128*)(size_t)byte_addr; }
 
 // Set bits of a wave mask in the lanes below this one (v_mbcnt: two
 // instructions, against an and and a popcount per half).
@@ -427,47 +442,77 @@ __device__ __forceinline__ uint64_t hit_key(float t, uint32_t p) {
 
 // Node phase: visits T.cur unless its entry distance lies beyond the closest
 // hit found since it was chosen, descends into the nearest hit internal
-// child or pops (T.cur = -1 when no node is left). Returns the hit leaf
-// children as a slot mask, with the node's children in ch.
+// child or pops (T.cur = -1 when no node is left). Returns the visit's leaf
+// group: slot c holds `owner | ~child` for a hit leaf child (owner: bit 31
+// and the lane in bits 25..30), else 0.
 template <int LS, bool COUNT>
-__device__ __forceinline__ uint32_t node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, int4& ch,
-                                                counters& C) {
-  uint32_t leaves = 0;
+__device__ __forceinline__ uint4 node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, uint32_t owner,
+                                             counters& C) {
+  uint4 g = make_uint4(0u, 0u, 0u, 0u);
   if (T.curd <= T.tbest) {
     node4_hits H;
     node4_boxes<COUNT>(P, T, H, C);
-    leaves = leaf_mask(H);
-    ch = H.ch;
-    if (order_push<LS, false>(P, T, H, ovf)) return leaves;
+    const int4 ch = H.ch;
+    g.x = (H.hit[0] && ch.x < 0) ? (owner | ~(uint32_t)ch.x) : 0u;
+    g.y = (H.hit[1] && ch.y < 0) ? (owner | ~(uint32_t)ch.y) : 0u;
+    g.z = (H.hit[2] && ch.z < 0) ? (owner | ~(uint32_t)ch.z) : 0u;
+    g.w = (H.hit[3] && ch.w < 0) ? (owner | ~(uint32_t)ch.w) : 0u;
+    if (order_push<LS, false>(P, T, H, ovf)) return g;
   }
   if (!pop4<LS>(P, T, ovf)) T.cur = -1;
-  return leaves;
+  return g;
 }
 
-// Leaf phase over the list entries [head, head + take) (ring positions),
-// take <= 64; runs with every lane of the wave active (ds_bpermute reads
-// the owners' registers).
+// Leaf phase over the ring's oldest groups (head group `head`, `avail` groups
+// listed): takes the first groups whose leaves fit 64 lanes (at least one)
+// and tests one leaf per lane; returns the groups taken, with their leaves in
+// `took`. Runs with every lane of the wave active (ds_bpermute reads the
+// owners' registers).
 template <bool COUNT>
-__device__ __forceinline__ void leaf_phase(const tparams& P, const f3 ro, const f3 rd, trav_state& T, uint32_t list,
-                                           uint32_t keys, uint32_t head, uint32_t take, uint32_t lane,
-                                           counters& C) {
+__device__ __forceinline__ uint32_t leaf_phase(const tparams& P, const f3 ro, const f3 rd, trav_state& T,
+                                               uint32_t list, uint32_t keys, uint32_t head, uint32_t avail,
+                                               uint32_t lane, uint32_t& took, counters& C) {
   lds_u64* mine = lds_dword(keys + lane * 8u);
   *mine = hit_key(T.tbest, T.found ? T.pbest : 0xffffffffu);
+  uint4 g = make_uint4(0u, 0u, 0u, 0u);
+  if (lane < avail && lane < 64u) {
+    const u32x4 v = *lds_quad(list + ((head + lane) & (kLeafGroups - 1u)) * 16u);
+    g = make_uint4(v.x, v.y, v.z, v.w);
+  }
+  const uint32_t n0 = g.x != 0u, n1 = g.y != 0u, n2 = g.z != 0u;
+  const uint32_t n = n0 + n1 + n2 + (g.w != 0u ? 1u : 0u);
+  // exclusive scan of the groups' leaf counts (0..4) from three ballots
+  const unsigned long long b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
+  const uint32_t before = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
+  // the scan is nondecreasing in the lane, so the groups that fit are a prefix
+  const unsigned long long fit = __ballot(before + n <= 64u && lane < avail);
+  const uint32_t ng = (uint32_t)__popcll(fit);
+  const uint32_t take =
+      (uint32_t)(__popcll(b0 & fit) + 2 * __popcll(b1 & fit) + 4 * __popcll(b2 & fit));
+  // the taken groups' leaf words, compacted in place over the groups' own
+  // words: leaf t goes to word t of the taken region, and t < 4 * (group + 1)
+  // never reaches a word of a later group before that group's lane has read it
+  // (all reads above completed before any write below)
+  const uint32_t w0 = head * 4u;
+  wave_lds_sync();
+  if ((fit >> lane) & 1ull) {
+    uint32_t t = w0 + before;
+    if (g.x) { *lds_word(list + (t & (kLeafWords - 1u)) * 4u) = g.x; ++t; }
+    if (g.y) { *lds_word(list + (t & (kLeafWords - 1u)) * 4u) = g.y; ++t; }
+    if (g.z) { *lds_word(list + (t & (kLeafWords - 1u)) * 4u) = g.z; ++t; }
+    if (g.w) { *lds_word(list + (t & (kLeafWords - 1u)) * 4u) = g.w; }
+  }
   wave_lds_sync();
   const bool tester = lane < take;
-  uint64_t e = 0;
-  if (tester) {
-    uint32_t pos = head + lane;
-    if (pos >= kLeafList) pos -= kLeafList;
-    e = *lds_dword(list + pos * 8u);
-  }
-  const uint32_t owner = (uint32_t)(e >> 32);
+  uint32_t e = 0u;
+  if (tester) e = *lds_word(list + ((w0 + lane) & (kLeafWords - 1u)) * 4u);
+  const uint32_t owner = (e >> kLeafPrimBits) & 63u;
   const int src = (int)(owner << 2);
   const f3 o = make(lane_value(src, ro.x), lane_value(src, ro.y), lane_value(src, ro.z));
   const f3 d = make(lane_value(src, rd.x), lane_value(src, rd.y), lane_value(src, rd.z));
   const float tn = lane_value(src, T.tnear);
   if (tester) {
-    const uint32_t p = (uint32_t)e;
+    const uint32_t p = e & kLeafPrimMask;
     float th;
     if (prim_hit<COUNT>(P, o, d, tn, p, th, C))
       __hip_atomic_fetch_min(lds_dword(keys + owner * 8u), hit_key(th, p), __ATOMIC_RELAXED,
@@ -478,6 +523,8 @@ __device__ __forceinline__ void leaf_phase(const tparams& P, const f3 ro, const 
   T.tbest = __uint_as_float((uint32_t)(k >> 32));
   T.pbest = (uint32_t)k;
   T.found = T.pbest != 0xffffffffu;
+  took = take;
+  return ng;
 }
 
 // Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
@@ -502,8 +549,8 @@ struct lane_stack<LS, 4> {
     return trav4_step<LS, COUNT>(P, o, d, T, ovf, C);
   }
   template <bool COUNT>
-  __device__ __forceinline__ uint32_t visit(const tparams& P, trav_state& T, int4& ch, counters& C) {
-    return node4_visit<LS, COUNT>(P, T, ovf, ch, C);
+  __device__ __forceinline__ uint4 visit(const tparams& P, trav_state& T, uint32_t owner, counters& C) {
+    return node4_visit<LS, COUNT>(P, T, ovf, owner, C);
   }
 };
 
@@ -873,11 +920,12 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
   // does not change any result.
   unsigned long long res_base = 0, res_end = 0;  // reserved ids [res_base, res_end), wave-uniform
   const unsigned long long tail = (unsigned long long)gridDim.x * (P.block >> 6) * 64ull * 16ull;
-  // the wave's leaf list (postponed leaves): wave-uniform head position in
-  // the ring, head sequence number and entry count
+  // the wave's leaf list (postponed leaves): wave-uniform head group of the
+  // ring, its sequence number, groups and leaves listed
   const uint32_t list = PL ? leaf_list_base<LS>(P.block) : 0u;
   const uint32_t keys = PL ? hit_key_base<LS>(P.block) : 0u;
-  uint32_t lhead = 0, lseq = 0, lcount = 0;
+  const uint32_t owner = 0x80000000u | ((uint32_t)lane << kLeafPrimBits);
+  uint32_t lhead = 0, lseq = 0, lcount = 0, lleaves = 0;
   for (;;) {
     // ended rays are replaced once refill_min lanes wait for one (a refill
     // costs the wave the same whatever the number of lanes it serves), or
@@ -929,41 +977,33 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
       unsigned long long tr = __ballot(tracing);
       for (;;) {
         const uint32_t nl = (uint32_t)__popcll(tr & __ballot(T.cur >= 0));
-        const uint32_t take = lcount < 64u ? lcount : 64u;
+        const uint32_t take = lleaves < 64u ? lleaves : 64u;
         const bool leaf_phase_now =
-            lcount != 0u && (nl == 0u || lcount + 4u * nl > kLeafList || take * 64u >= (uint32_t)P.leaf_min * nl);
+            lcount != 0u && (nl == 0u || lcount + nl > kLeafGroups || take * 64u >= (uint32_t)P.leaf_min * nl);
         if (leaf_phase_now) {
-          leaf_phase<COUNT>(P, R.o, R.d, T, list, keys, lhead, take, (uint32_t)lane, C);
-          lhead += take;
-          if (lhead >= kLeafList) lhead -= kLeafList;
-          lseq += take;
-          lcount -= take;
+          uint32_t took;
+          const uint32_t ng = leaf_phase<COUNT>(P, R.o, R.d, T, list, keys, lhead, lcount, (uint32_t)lane, took, C);
+          lhead = (lhead + ng) & (kLeafGroups - 1u);
+          lseq += ng;
+          lcount -= ng;
+          lleaves -= took;
         } else {
-          uint32_t lm = 0;
-          int4 ch = make_int4(0, 0, 0, 0);
-          if (tracing && T.cur >= 0) lm = S.template visit<COUNT>(P, T, ch, C);
-          // append the hit leaves: exclusive scan of the counts (0..4)
-          const uint32_t n = (uint32_t)__popc(lm);
-          const unsigned long long b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
-          const uint32_t before = lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2);
-          const uint32_t added = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
-          if (lm) {
-            uint32_t pos = lhead + lcount;  // the list's end: lhead, lcount < kLeafList here
-            if (pos >= kLeafList) pos -= kLeafList;
-            pos += before;
-            const uint64_t own = (uint64_t)lane << 32;
-            T.lastq = lseq + lcount + before + n;
-            // one trip per hit leaf of the busiest lane (mostly one), not four
-            // predicated slots
-            do {
-              const uint32_t c = (uint32_t)(__ffs(lm) - 1);
-              lm &= lm - 1u;
-              if (pos >= kLeafList) pos -= kLeafList;
-              *lds_dword(list + pos * 8u) = own | (uint32_t)(~sel4(ch, c));
-              ++pos;
-            } while (lm);
+          uint4 g = make_uint4(0u, 0u, 0u, 0u);
+          if (tracing && T.cur >= 0) g = S.template visit<COUNT>(P, T, owner, C);
+          // append the visit's group: one 16-byte write per lane with hit leaves
+          const unsigned long long bx = __ballot(g.x != 0u), by = __ballot(g.y != 0u);
+          const unsigned long long bz = __ballot(g.z != 0u), bw = __ballot(g.w != 0u);
+          const unsigned long long bg = bx | by | bz | bw;
+          if ((g.x | g.y | g.z | g.w) != 0u) {
+            const uint32_t rank = lanes_below(bg);
+            // the list's end: lhead + lcount, lcount + nl <= kLeafGroups here
+            u32x4 v;
+            v.x = g.x; v.y = g.y; v.z = g.z; v.w = g.w;
+            *lds_quad(list + ((lhead + lcount + rank) & (kLeafGroups - 1u)) * 16u) = v;
+            T.lastq = lseq + lcount + rank + 1u;
           }
-          lcount += added;
+          lcount += (uint32_t)__popcll(bg);
+          lleaves += (uint32_t)(__popcll(bx) + __popcll(by) + __popcll(bz) + __popcll(bw));
         }
         const bool pend = (int32_t)(T.lastq - lseq) > 0;
         tr &= __ballot(T.cur >= 0) | __ballot(pend);
@@ -1057,7 +1097,7 @@ kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool a
 // lists and the threads' closest-hit keys.
 size_t trace_lds_bytes(int width, int block_size, int lds_stack, bool postpone) {
   size_t b = (size_t)block_size * lds_stack * (width == 4 ? 8 : 4);
-  if (width == 4 && postpone) b += (size_t)(block_size / 64) * kLeafList * 8 + (size_t)block_size * 8;
+  if (width == 4 && postpone) b += (size_t)(block_size / 64) * kLeafGroups * 16 + (size_t)block_size * 8;
   return b;
 }
 
@@ -1072,6 +1112,7 @@ int default_compact_lanes() { return 56; }
 // 64..192) but not stores of discs only, whose leaf test is a few
 // instructions (batch of disc clouds -2 %, rough surface -2 % at 64).
 int default_leaf_batch(bool triangles) { return triangles ? 128 : 0; }
+uint64_t postponed_leaf_prim_limit() { return 1ull << kLeafPrimBits; }
 int wide_stack_capacity() { return kWideStack; }
 
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
