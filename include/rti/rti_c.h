@@ -253,16 +253,21 @@ int rti_get_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint32_t* p
                       float* prim_a);
 
 /* The four-wide traversal tree of the (single) geometry: 64-byte records
- * { float org[3], s_x; uint32 lo_x, lo_y, lo_z, hi_x, hi_y, hi_z;
- *   float s_y, s_z; int32 child[4] },
- * child c's box planes fmaf(byte c of the plane word, |s|, org) with s a
- * power of two; nchild - 1 = (s_y < 0) | (s_z < 0) << 1; slots beyond
- * nchild hold an inverted box and a leaf of the node's subtree;
- * child >= 0 is a node index relative to this geometry, < 0 is ~stored
- * primitive position. Root is record 0. Each family of siblings starts on
- * an even record (a 128-byte line); records no node references are padding
- * (an inverted box in all four slots). The reference's Embree BVH is
- * internal to Embree and has no such export; this one serves validation. */
+ * { float org[3]; uint32 meta, lo_x, lo_y, lo_z, hi_x, hi_y, hi_z,
+ *   int_base, leaf_info, leaf[4] },
+ * meta = e_x | e_y << 8 | e_z << 16 | (nchild - 1) << 24 | nint << 26
+ *        (| 1 << 29 in a triangle mesh's tree: leaves contiguous);
+ * child c's box planes fmaf(byte c of the plane word, 2^(e - 127), org).
+ * Children are ordered internal first: slot c < nint is node int_base + c
+ * (node index relative to this geometry), slot nint <= c < nchild is the
+ * primitive at stored position leaf[c]; slots beyond nchild hold an inverted
+ * box. In a geometry of triangles the stored positions follow the tree
+ * depth first, each node's leaves contiguous from leaf_info; for discs
+ * (Morton order) bit c of leaf_info marks an internal child with leaf
+ * children. Root is record 0. Each family of siblings starts on an even record
+ * (a 128-byte line); records no node references are padding (an inverted
+ * box in all four slots). The reference's Embree BVH is internal to Embree
+ * and has no such export; this one serves validation. */
 int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64_t* n_nodes);
 
 /* Batch: many geometries in one persistent launch (captured in a HIP graph

@@ -82,15 +82,23 @@ class Stats(ctypes.Structure):
 # Four-wide traversal-tree record (64 bytes): child c's box planes are
 # fmaf(byte c of the plane word, |s|, org) with s the axis's power-of-two
 # scale; the signs of s_y and s_z hold nchild - 1 (bits 0 and 1).
-WIDE_NODE = np.dtype([("org", "<f4", 3), ("sx", "<f4"), ("lo", "<u4", 3), ("hi", "<u4", 3), ("syz", "<f4", 2),
-                      ("child", "<i4", 4)])
+WIDE_NODE = np.dtype([("org", "<f4", 3), ("meta", "<u4"), ("lo", "<u4", 3), ("hi", "<u4", 3), ("int_base", "<u4"),
+                      ("leaf_info", "<u4"), ("leaf", "<u4", 4)])
 
 
 def wide_scales(node):
-    """(|s_x|, |s_y|, |s_z|) and the child count of a four-wide record."""
-    s = np.abs(np.array([node["sx"], node["syz"][0], node["syz"][1]], dtype=np.float32))
-    m = 1 + int(node["syz"][0] < 0) + 2 * int(node["syz"][1] < 0)
-    return s, m
+    """(s_x, s_y, s_z) = 2^(e - 127) and the child count of a four-wide record."""
+    meta = int(node["meta"])
+    s = np.array([np.ldexp(1.0, ((meta >> (8 * k)) & 0xFF) - 127) for k in range(3)], dtype=np.float32)
+    return s, 1 + ((meta >> 24) & 3)
+
+
+def wide_children(node):
+    """Child references of a four-wide record in slot order: a node index
+    (>= 0) for internal children, ~stored position for leaves."""
+    meta = int(node["meta"])
+    m, nint = 1 + ((meta >> 24) & 3), (meta >> 26) & 7
+    return [int(node["int_base"]) + c if c < nint else ~int(node["leaf"][c]) for c in range(m)]
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -394,6 +394,8 @@ int prepare(rti_ctx* ctx) {
   }
   if (np >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives in total");
   if (na >= (1ull << 31)) return fail(ctx, RTI_ERR_INVALID_ARG, "too many primitives over all instances");
+  // four-wide node references keep bit 30 for the parent's leaf mark
+  if (nn4 >= rti_dev::kNodeHasLeaves) return fail(ctx, RTI_ERR_INVALID_ARG, "too many tree nodes in total");
   ctx->n_total = np;
   ctx->acc_total = na;
   ctx->nodes_total = nn;
@@ -602,6 +604,14 @@ int upload_descriptors(rti_ctx* ctx, inst_desc* dst) {
   return RTI_OK;
 }
 
+// Every instance of the context is a triangle mesh: each four-wide node
+// keeps its leaves contiguous, and the kernels need no leaf word.
+int all_meshes(const rti_ctx* ctx) {
+  for (const auto& I : ctx->inst)
+    if (I.kind != 1) return 0;
+  return 1;
+}
+
 struct kernel_cfg {
   int block, lds, grid, width;
   int leaf;  // postponed-leaf threshold (64ths), 0 = leaf tests at each node
@@ -638,7 +648,8 @@ int pick_config(rti_ctx* ctx, kernel_cfg& cfg, bool adj, bool fp64) {
                     rti_impl::wide_stack_capacity());
   }
   int per_cu = 0;
-  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, cfg.leaf, adj, fp64, cfg.width, &per_cu));
+  HIP_TRY(ctx, rti_impl::trace_occupancy(cfg.lds, cfg.block, 1, cfg.leaf, adj, fp64, cfg.width, all_meshes(ctx),
+                                         &per_cu));
   if (per_cu < 1) per_cu = 1;
   if (p.waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, p.waves_per_cu * 64 / cfg.block));
   cfg.grid = per_cu * ctx->num_cus;
@@ -702,6 +713,7 @@ int enqueue_trace(rti_ctx* ctx, const trace_bufs& B, const kernel_cfg& cfg, bool
   L.width = cfg.width;
   L.leaf_batch = cfg.leaf;
   L.refill_lanes = ctx->params.refill_lanes ? ctx->params.refill_lanes : 1;
+  L.mesh = all_meshes(ctx);
   if (kernel == RTI_KERNEL_PERSISTENT) {
     L.ray_lo = 0;
     L.ray_hi = ctx->rays_total;
@@ -1197,6 +1209,13 @@ int rti_get_wide_hierarchy(rti_ctx* ctx, void* nodes, uint64_t cap_nodes, uint64
     if (cap_nodes < I.n_wide) return fail(ctx, RTI_ERR_INVALID_ARG, "node capacity too small");
     HIP_TRY(ctx, hipMemcpy(nodes, ctx->nodes4.as<bvh4_node>() + I.node4_begin, I.n_wide * sizeof(bvh4_node),
                            hipMemcpyDeviceToHost));
+    bvh4_node* w = static_cast<bvh4_node*>(nodes);
+    for (uint32_t i = 0; i < I.n_wide; ++i) {  // relative to this geometry
+      const int m = rti_dev::bvh4_nchild(w[i]), nint = rti_dev::bvh4_nint(w[i]);
+      w[i].int_base -= nint ? I.node4_begin : w[i].int_base;
+      for (int c = nint; c < m; ++c) w[i].leaf[c] -= I.prim_begin;
+      if (w[i].meta & rti_dev::kNodeLeavesContiguous) w[i].leaf_info -= I.prim_begin;
+    }
   }
   return RTI_OK;
 }

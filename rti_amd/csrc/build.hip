@@ -575,25 +575,36 @@ __global__ void k_c4_count(const int* items, uint32_t width, const bvh_node* nod
 // items[i] (binary node) becomes four-wide slot node4_begin + base + i; its
 // internal children get slots node4_begin + base + width + offsets[i] + j.
 // A padding item (< 0) becomes a node whose four slots all hold an inverted
-// box and the leaf pad_leaf.
+// box. pad[base + i] marks the padding slots for the stack bound.
 __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, const bvh_node* nodes,
-                           const uint32_t* offsets, uint32_t node4_begin, int pad_leaf, bvh4_node* nodes4,
-                           int* next) {
+                           const uint32_t* offsets, uint32_t node4_begin, bvh4_node* nodes4, int* next,
+                           uint32_t* pad_flag) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= width) return;
+  pad_flag[base + i] = items[i] < 0 ? 1u : 0u;
   if (items[i] < 0) {
     bvh4_node pad;
     pad.ox = pad.oy = pad.oz = 0.0f;
-    pad.sx = pad.sy = pad.sz = 1.0f;
+    pad.meta = 127u | (127u << 8) | (127u << 16);  // unit scales, one child slot, no internal child
     pad.lo_x = pad.lo_y = pad.lo_z = 0xffffffffu;
     pad.hi_x = pad.hi_y = pad.hi_z = 0u;
-    pad.child = make_int4(pad_leaf, pad_leaf, pad_leaf, pad_leaf);
+    pad.int_base = 0u;
+    pad.leaf_info = 0u;
+    pad.leaf[0] = pad.leaf[1] = pad.leaf[2] = pad.leaf[3] = 0u;
     nodes4[node4_begin + base + i] = pad;
     return;
   }
+  int ref0[4];
+  c4_box box0[4];
+  const int m = c4_members(nodes, items[i], ref0, box0);
+  // internal children first, then the leaves, each in left-to-right order
   int ref[4];
   c4_box box[4];
-  const int m = c4_members(nodes, items[i], ref, box);
+  int nint = 0;
+  for (int c = 0; c < m; ++c)
+    if (ref0[c] >= 0) { ref[nint] = ref0[c]; box[nint] = box0[c]; ++nint; }
+  for (int c = 0, k = nint; c < m; ++c)
+    if (ref0[c] < 0) { ref[k] = ref0[c]; box[k] = box0[c]; ++k; }
   float lo[3], hi[3];
   for (int k = 0; k < 3; ++k) {
     lo[k] = box[0].lo[k];
@@ -610,69 +621,143 @@ __global__ void k_c4_write(const int* items, uint32_t width, uint32_t base, cons
     sc[k] = __uint_as_float(e[k] << 23);
   }
   // Slots beyond the m children hold an inverted box (low planes 255, high
-  // planes 0), which no slab test enters, and a primitive of this subtree as
-  // a leaf, so that even a test that rounded into one would only test a
-  // primitive of the ray's own geometry again (never push a stack entry the
-  // stack bound does not count).
+  // planes 0), which no slab test enters.
   uint32_t ql[3] = {0, 0, 0}, qh[3] = {0, 0, 0};
   for (int c = m; c < 4; ++c)
     for (int k = 0; k < 3; ++k) ql[k] |= 0xffu << (8 * c);
-  int child[4] = {0, 0, 0, 0};
-  uint32_t j = 0;
   const uint32_t first = base + width + offsets[i];
   for (int c = 0; c < m; ++c) {
     for (int k = 0; k < 3; ++k) {
       ql[k] |= c4_qlo(box[c].lo[k], lo[k], sc[k]) << (8 * c);
       qh[k] |= c4_qhi(box[c].hi[k], lo[k], sc[k]) << (8 * c);
     }
-    if (ref[c] >= 0) {
-      next[offsets[i] + j] = ref[c];
-      child[c] = (int)(node4_begin + first + j);
-      ++j;
-    } else {
-      child[c] = ref[c];
-    }
+    if (c < nint) next[offsets[i] + c] = ref[c];
   }
-  if (j & 1u) next[offsets[i] + j] = -1;  // the family's padding slot
+  if (nint & 1) next[offsets[i] + nint] = -1;  // the family's padding slot
   bvh4_node out;
   out.ox = lo[0];
   out.oy = lo[1];
   out.oz = lo[2];
-  out.sx = sc[0];
-  out.sy = ((m - 1) & 1) ? -sc[1] : sc[1];
-  out.sz = ((m - 1) & 2) ? -sc[2] : sc[2];
+  out.meta = e[0] | (e[1] << 8) | (e[2] << 16) | ((uint32_t)(m - 1) << 24) | ((uint32_t)nint << 26);
   out.lo_x = ql[0];
   out.lo_y = ql[1];
   out.lo_z = ql[2];
   out.hi_x = qh[0];
   out.hi_y = qh[1];
   out.hi_z = qh[2];
-  int spare = ref[0];
-  while (spare >= 0) spare = nodes[spare].child.x;
-  for (int c = m; c < 4; ++c) child[c] = spare;
-  out.child = make_int4(child[0], child[1], child[2], child[3]);
+  out.int_base = node4_begin + first;
+  out.leaf_info = 0u;  // set by k_c4_stack (discs) or k_c4_place (triangles)
+  for (int c = 0; c < 4; ++c) out.leaf[c] = (c >= nint && c < m) ? (uint32_t)(~ref[c]) : 0u;
   nodes4[node4_begin + base + i] = out;
 }
 
-// Stack bound, deepest level first: need(v) = k - 1 + max need(internal
-// child) for a node with k >= 1 internal children, 0 otherwise. A traversal
-// that pushes all hit internal children but the one it descends into never
-// holds more than need(root) entries.
-__global__ void k_c4_stack(const bvh4_node* nodes4, uint32_t node4_begin, uint32_t base, uint32_t width,
-                           uint32_t* need) {
+// Stack bound and primitive counts, deepest level first: need(v) = k - 1 +
+// max need(internal child) for a node with k >= 1 internal children, 0
+// otherwise (a traversal that pushes all hit internal children but the one
+// it descends into never holds more than need(root) entries); tot(v) = the
+// primitives below v. Padding slots count nothing. Also marks the internal
+// children that have leaf children (leaf_info; a triangle mesh replaces it
+// with the node's first leaf in k_c4_place).
+__global__ void k_c4_stack(bvh4_node* nodes4, const uint32_t* pad_flag, uint32_t node4_begin, uint32_t base,
+                           uint32_t width, uint32_t* need, uint32_t* tot) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= width) return;
-  const bvh4_node nd = nodes4[node4_begin + base + i];
-  const int m = bvh4_nchild(nd);
-  const int ch[4] = {nd.child.x, nd.child.y, nd.child.z, nd.child.w};
-  uint32_t k = 0, deepest = 0;
-  for (int c = 0; c < m; ++c) {
-    if (ch[c] < 0) continue;
-    ++k;
-    deepest = max(deepest, need[(uint32_t)ch[c] - node4_begin]);
+  if (pad_flag[base + i]) {
+    need[base + i] = 0u;
+    tot[base + i] = 0u;
+    return;
   }
-  need[base + i] = k ? (k - 1 + deepest) : 0u;
+  bvh4_node* nd = nodes4 + node4_begin + base + i;
+  const int nint = bvh4_nint(*nd);
+  uint32_t deepest = 0, marks = 0, t = (uint32_t)(bvh4_nchild(*nd) - nint);
+  for (int c = 0; c < nint; ++c) {
+    const uint32_t v = nd->int_base + (uint32_t)c - node4_begin;
+    const bvh4_node& ch = nodes4[nd->int_base + (uint32_t)c];
+    deepest = max(deepest, need[v]);
+    t += tot[v];
+    if (bvh4_nchild(ch) > bvh4_nint(ch)) marks |= 1u << c;
+  }
+  nd->leaf_info = marks;
+  need[base + i] = nint ? ((uint32_t)nint - 1u + deepest) : 0u;
+  tot[base + i] = t;
 }
+
+// Leaf placement of a triangle mesh, root level first: node v's
+// leaves take the stored positions [off(v), off(v) + leaves of v), its
+// internal children's subtrees follow in slot order (a depth-first order of
+// the tree, so each node's leaves are contiguous). newpos[old local
+// position] = new local position; the node's leaf_info and leaf[] get the
+// new positions. The binary tree still holds the old positions here.
+__global__ void k_c4_place(bvh4_node* nodes4, const uint32_t* pad_flag, uint32_t node4_begin, uint32_t prim_begin,
+                           uint32_t base, uint32_t width, const uint32_t* tot, uint32_t* off, uint32_t* newpos) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= width) return;
+  if (pad_flag[base + i]) return;
+  bvh4_node* nd = nodes4 + node4_begin + base + i;
+  const int m = bvh4_nchild(*nd), nint = bvh4_nint(*nd);
+  const uint32_t o = off[base + i];
+  nd->leaf_info = prim_begin + o;
+  nd->meta |= rti_dev::kNodeLeavesContiguous;
+  uint32_t run = o + (uint32_t)(m - nint);
+  for (int c = 0; c < nint; ++c) {
+    const uint32_t v = nd->int_base + (uint32_t)c - node4_begin;
+    off[v] = run;
+    run += tot[v];
+  }
+  for (int c = nint; c < m; ++c) {
+    newpos[nd->leaf[c] - prim_begin] = o + (uint32_t)(c - nint);
+    nd->leaf[c] = prim_begin + o + (uint32_t)(c - nint);
+  }
+}
+
+// Records, input ids and areas of one instance to their new stored
+// positions: gather into scratch (old order), then scatter.
+__global__ void k_perm_gather(const float4* prims, uint32_t shift, const uint32_t* perm, const float* areas,
+                              uint32_t prim_begin, uint32_t n, float4* t0, float4* t1, float4* t2, float4* t3,
+                              uint32_t* tperm, float* tarea) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float4* rec = prims + ((size_t)(prim_begin + p) << shift);
+  t0[p] = rec[0];
+  t1[p] = rec[1];
+  if (shift == 2) {
+    t2[p] = rec[2];
+    t3[p] = rec[3];
+  }
+  tperm[p] = perm[prim_begin + p];
+  tarea[p] = areas[prim_begin + p];
+}
+
+__global__ void k_perm_scatter(float4* prims, uint32_t shift, uint32_t* perm, float* areas, uint32_t prim_begin,
+                               uint32_t n, const uint32_t* newpos, const float4* t0, const float4* t1,
+                               const float4* t2, const float4* t3, const uint32_t* tperm, const float* tarea) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (newpos[p] >= n) return;  // cannot happen: newpos is a permutation of [0, n)
+  const uint32_t q = prim_begin + newpos[p];
+  float4* rec = prims + ((size_t)q << shift);
+  rec[0] = t0[p];
+  rec[1] = t1[p];
+  if (shift == 2) {
+    rec[2] = t2[p];
+    rec[3] = t3[p];
+  }
+  perm[q] = tperm[p];
+  areas[q] = tarea[p];
+}
+
+// Leaf references of the binary tree to the new stored positions.
+__global__ void k_remap_binary(bvh_node* nodes, uint32_t node_begin, uint32_t count, uint32_t prim_begin,
+                               const uint32_t* newpos) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int4 ch = nodes[node_begin + i].child;
+  if (ch.x < 0) ch.x = ~(int)(prim_begin + newpos[(uint32_t)(~ch.x) - prim_begin]);
+  if (ch.y < 0) ch.y = ~(int)(prim_begin + newpos[(uint32_t)(~ch.y) - prim_begin]);
+  nodes[node_begin + i].child = ch;
+}
+
+__global__ void k_set_first_u32(uint32_t* p, uint32_t v) { p[0] = v; }
 
 // The root level: the binary root and one padding slot.
 __global__ void k_c4_root(int* items, uint32_t node_begin) {
@@ -1019,9 +1104,12 @@ hipError_t build_instance(const build_job& job, const store_ptrs& st, build_scra
 hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch& s, uint32_t* n_wide,
                       uint32_t* stack_bound, hipStream_t stream) {
   // work items ping-pong in keys[0] / keys[1], counts in vals[0], child
-  // offsets in vals[1], stack needs in flags (all at least n entries)
+  // offsets in vals[1], stack needs in flags, padding marks in parent (as
+  // words; all at least wide_capacity(n) entries, the binary build is done
+  // with parent)
   const unsigned T = 256;
   int* items[2] = {reinterpret_cast<int*>(s.keys[0]), reinterpret_cast<int*>(s.keys[1])};
+  uint32_t* pad_flag = reinterpret_cast<uint32_t*>(s.parent);
   k_c4_root<<<1, 1, 0, stream>>>(items[0], job.node_begin);
   std::vector<std::pair<uint32_t, uint32_t>> levels;  // (base, width)
   uint32_t base = 0, width = 2;
@@ -1033,8 +1121,7 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
     e = exclusive_scan_u32(s.vals[0], s.vals[1], width + 1, s.scan_tmp, stream);
     if (e != hipSuccess) return e;
     k_c4_write<<<blocks_for(width, T), T, 0, stream>>>(items[cur], width, base, st.nodes, s.vals[1],
-                                                       job.node4_begin, ~(int)job.prim_begin, st.nodes4,
-                                                       items[cur ^ 1]);
+                                                       job.node4_begin, st.nodes4, items[cur ^ 1], pad_flag);
     uint32_t next = 0;
     e = hipMemcpyAsync(&next, s.vals[1] + width, 4, hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return e;
@@ -1045,9 +1132,40 @@ hipError_t build_wide(const build_job& job, const store_ptrs& st, build_scratch&
     cur ^= 1;
     if (base + width > wide_capacity(job.n)) return hipErrorInvalidValue;  // cannot happen
   }
+  uint32_t* tot = reinterpret_cast<uint32_t*>(s.leaf_lo);
   for (size_t l = levels.size(); l-- > 0;)
-    k_c4_stack<<<blocks_for(levels[l].second, T), T, 0, stream>>>(st.nodes4, job.node4_begin, levels[l].first,
-                                                                   levels[l].second, s.flags);
+    k_c4_stack<<<blocks_for(levels[l].second, T), T, 0, stream>>>(st.nodes4, pad_flag, job.node4_begin,
+                                                                   levels[l].first, levels[l].second, s.flags, tot);
+  if (job.prim_kind == 1) {
+    // a triangle mesh: primitives in the tree's depth-first order
+    // (leaf totals and offsets in leaf_lo / leaf_hi as words, new positions in
+    // counts, the records' staging in clo / chi / keys; the binary build is
+    // done with all of them)
+    uint32_t* off = reinterpret_cast<uint32_t*>(s.leaf_hi);
+    uint32_t* newpos = s.counts;
+    // every primitive is the leaf of exactly one node: the root's subtree
+    // holds all n (the placement then writes a permutation of [0, n))
+    uint32_t total = 0;
+    e = hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    if (total != job.n) return hipErrorInvalidValue;  // cannot happen
+    k_set_first_u32<<<1, 1, 0, stream>>>(off, 0u);
+    for (size_t l = 0; l < levels.size(); ++l)
+      k_c4_place<<<blocks_for(levels[l].second, T), T, 0, stream>>>(st.nodes4, pad_flag, job.node4_begin,
+                                                                     job.prim_begin, levels[l].first,
+                                                                     levels[l].second, tot, off, newpos);
+    const uint32_t n = job.n;
+    k_perm_gather<<<blocks_for(n, T), T, 0, stream>>>(st.prims, st.prim_shift, st.perm, st.areas, job.prim_begin,
+                                                      n, s.clo[0], s.clo[1], s.chi[0], s.chi[1], s.keys[0],
+                                                      reinterpret_cast<float*>(s.keys[1]));
+    k_perm_scatter<<<blocks_for(n, T), T, 0, stream>>>(st.prims, st.prim_shift, st.perm, st.areas, job.prim_begin,
+                                                       n, newpos, s.clo[0], s.clo[1], s.chi[0], s.chi[1], s.keys[0],
+                                                       reinterpret_cast<const float*>(s.keys[1]));
+    const uint32_t nbin = n > 1 ? n - 1 : 1;
+    k_remap_binary<<<blocks_for(nbin, T), T, 0, stream>>>(st.nodes, job.node_begin, nbin, job.prim_begin, newpos);
+  }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(stack_bound, s.flags, 4, hipMemcpyDeviceToHost, stream);

@@ -116,10 +116,11 @@ struct trace_launch {
                                     // phase once the entries it takes reach this many 64ths of the lanes with
                                     // a node left (1..256); 0 = at each node
   int refill_lanes;                 // persistent: lanes waiting for a new ray before a wave refills (1..64)
+  int mesh;                         // every instance is a triangle mesh (four-wide nodes without a leaf word)
 };
 hipError_t launch_trace(const trace_launch& L, hipStream_t stream);
 hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,
-                           int width, int* blocks_per_cu);
+                           int width, int mesh, int* blocks_per_cu);
 int default_lds_stack(int width, bool postpone);
 // Stack entries (LDS + private) a four-wide traversal holds at most.
 int wide_stack_capacity();

@@ -30,9 +30,11 @@
 //                  its primitive records and trees but not their slots
 //   adj[2a + k]    u64  adjoint sums d/ds (k = 0) and d/dn (k = 1), signed fixed point
 //   mom[2a + k]    f64  sums of deposit^3 (k = 0) and deposit^4 (k = 1)
-// Primitives are stored in Morton order of their instance, so neighbouring
-// primitives in space are neighbours in memory (coalesced leaf and
-// accumulator traffic); `perm` maps a stored position back to the input id.
+// Primitives are stored in Morton order of their instance (disc clouds) or
+// in the depth-first order of its four-wide tree over Morton order (each
+// node's leaves contiguous; triangle meshes), so neighbouring primitives in
+// space are neighbours in memory (local leaf and accumulator traffic);
+// `perm` maps a stored position back to the input id.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,27 +60,39 @@ static_assert(sizeof(bvh_node) == 64, "node must be one 64-byte record");
 
 // Four-wide node, collapsed from the binary tree, with its children's boxes
 // quantised to 8 bits per plane in the frame of the node's own box:
-//   plane = fmaf(q, |s_axis|, org_axis),
-// where q is byte c (child c) of the plane's word and s_axis = +-2^k the
-// axis's scale, stored as a float so that the traversal scales it by the
-// ray's reciprocal direction in one multiply (the sign goes as a source
-// modifier). The build rounds lower planes down and upper planes up through
-// exactly this expression, so every quantised box contains the child's exact
-// box. The signs of s_y and s_z hold nchild - 1 (bit 0 and bit 1). Children
-// 0..nchild-1 are valid; slots beyond hold an inverted box (low bytes 255,
-// high bytes 0) and a leaf of the node's subtree. child[c] >= 0 is a
-// four-wide node index, child[c] < 0 is leaf ~prim. 64 bytes.
+//   plane = fmaf(q, 2^(e_axis - 127), org_axis),
+// where q is byte c (child c) of the plane's word and e_axis the axis's
+// biased exponent. The build rounds lower planes down and upper planes up
+// through exactly this expression, so every quantised box contains the
+// child's exact box. Children are ordered internal first: slots 0..nint-1
+// are the four-wide nodes int_base + c (siblings are contiguous), slots
+// nint..nchild-1 are leaves, primitive leaf[c] (stored position); slots
+// nchild..3 hold an inverted box (low bytes 255, high 0) and no child.
+// 64 bytes; a visit loads the first three 16-byte words (boxes, internal
+// children and leaf_info). The primitives of a triangle mesh are stored in
+// its tree's depth-first order, each node's leaves contiguous
+// (kNodeLeavesContiguous in meta), so leaf c is leaf_info + c - nint and the
+// fourth word is never loaded; in the tree of a disc cloud (Morton order)
+// leaf_info marks the internal children that have leaf children, and the
+// visit of a marked node loads the fourth word with the other three.
 struct __align__(16) bvh4_node {
-  float ox, oy, oz, sx;
+  float ox, oy, oz;
+  uint32_t meta;       // e_x | e_y << 8 | e_z << 16 | (nchild - 1) << 24 | nint << 26 | kNodeLeavesContiguous
   uint32_t lo_x, lo_y, lo_z, hi_x;
   uint32_t hi_y, hi_z;
-  float sy, sz;
-  int4 child;
+  uint32_t int_base;   // first internal child (absolute four-wide index)
+  uint32_t leaf_info;  // triangles: first leaf (absolute stored position); discs: bit c = internal
+                       // child c has leaf children
+  uint32_t leaf[4];    // leaf slots: absolute stored position of the primitive; other slots 0
 };
+constexpr uint32_t kNodeLeavesContiguous = 1u << 29;
+// A reference to a four-wide node in the traversal of a tree of discs
+// carries, in this bit, its parent's mark that the node has leaf children
+// (node indices stay below it).
+constexpr uint32_t kNodeHasLeaves = 1u << 30;
 
-__host__ __device__ inline int bvh4_nchild(const bvh4_node& nd) {
-  return 1 + (nd.sy < 0.0f ? 1 : 0) + (nd.sz < 0.0f ? 2 : 0);
-}
+__host__ __device__ inline int bvh4_nchild(const bvh4_node& nd) { return 1 + (int)((nd.meta >> 24) & 3u); }
+__host__ __device__ inline int bvh4_nint(const bvh4_node& nd) { return (int)((nd.meta >> 26) & 7u); }
 static_assert(sizeof(bvh4_node) == 64, "four-wide node must be one 64-byte record");
 
 // Per-instance description, read by each lane for the instance its ray

@@ -209,12 +209,6 @@ __device__ __forceinline__ lds_u64* lds_slot(uint32_t j, uint32_t stride) {
   return (lds_u64*)(trace_lds) + (__umul24(j, stride) + threadIdx.x);
 }
 
-__device__ __forceinline__ uint32_t sel4(int4 v, uint32_t c) {
-  const int lo = (c & 1u) ? v.y : v.x;
-  const int hi = (c & 1u) ? v.w : v.z;
-  return (uint32_t)((c & 2u) ? hi : lo);
-}
-
 // tmin >= tnear > 0, so the float's bits order like the float; the low two
 // bits carry the child slot (ties go to the lower slot).
 __device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
@@ -222,8 +216,8 @@ __device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
 }
 
 // Child boxes of the four-wide node T.cur: entry distance and hit (slab
-// interval non-empty within [tnear, tbest]) per child. Empty slots hold an
-// inverted box, which no ray enters.
+// interval non-empty within [tnear, tbest]) per child, and where its
+// children are. Empty slots hold an inverted box, which no ray enters.
 //
 // Slab distances: the plane of byte q is org + q * 2^e, and
 //   t = (org + q * 2^e) * inv - o * inv = q * (2^e * inv) + (org * inv - o * inv),
@@ -231,19 +225,50 @@ __device__ __forceinline__ uint32_t sort_key(bool hit, float tmin, uint32_t c) {
 struct node4_hits {
   float tmin[4];
   bool hit[4];
-  int4 ch;
+  uint32_t int_base;   // internal child c (c < nint): node int_base + c
+  uint32_t leaf_info;  // triangle stores: first leaf; disc stores: internal children with leaves
+  uint32_t nint, nchild;
+  bool contiguous;     // leaves contiguous from leaf_info (a tree of triangles)
+  uint4 lp;            // the leaf slots' primitives (a node of a disc cloud's tree)
 };
 
-template <bool COUNT>
+// Primitive of leaf slot c (nint <= c < nchild).
+__device__ __forceinline__ uint32_t leaf_prim(const node4_hits& H, uint32_t c) {
+  if (H.contiguous) return H.leaf_info + c - H.nint;
+  return c == 0 ? H.lp.x : c == 1 ? H.lp.y : c == 2 ? H.lp.z : H.lp.w;
+}
+
+// Reference to internal child `slot`: its index and, in a tree of discs, the
+// parent's mark that it has leaf children.
+__device__ __forceinline__ uint32_t child_ref(const node4_hits& H, bool contiguous, uint32_t slot) {
+  const uint32_t mark = contiguous ? 0u : ((H.leaf_info >> slot) & 1u) << 30;
+  return (H.int_base + slot) | mark;
+}
+
+// MESH: a launch over triangle meshes only, whose nodes all keep their
+// leaves contiguous (no fourth word, no marks).
+template <bool COUNT, bool MESH>
 __device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& T, node4_hits& H, counters& C) {
-  const uint4* q = reinterpret_cast<const uint4*>(P.nodes4 + T.cur);
+  const uint4* q = reinterpret_cast<const uint4*>(
+      P.nodes4 + (MESH ? (uint32_t)T.cur : ((uint32_t)T.cur & (kNodeHasLeaves - 1u))));
   const uint4 q0 = q[0], q1 = q[1], q2 = q[2];
-  H.ch = reinterpret_cast<const int4*>(q)[3];
+  // the fourth word only for a node its parent marked (trees of discs)
+  H.lp = make_uint4(0u, 0u, 0u, 0u);
+  if (!MESH && ((uint32_t)T.cur & kNodeHasLeaves)) H.lp = q[3];
   if (COUNT) C.nodes++;
+  const uint32_t meta = q0.w;
+  H.int_base = q2.z;
+  H.leaf_info = q2.w;
+  H.nint = (meta >> 26) & 7u;
+  H.nchild = 1u + ((meta >> 24) & 3u);
+  H.contiguous = MESH || (meta & kNodeLeavesContiguous) != 0u;
+
+  // 2^(e - 127) from each biased exponent byte (meta << 23 puts e_x in the
+  // exponent field and one bit of e_y in the sign)
   const f3 inv = T.inv, oi = T.oi;
-  const float ax = fabsf(__uint_as_float(q0.w)) * inv.x;
-  const float ay = fabsf(__uint_as_float(q2.z)) * inv.y;
-  const float az = fabsf(__uint_as_float(q2.w)) * inv.z;
+  const float ax = fabsf(__uint_as_float(meta << 23)) * inv.x;
+  const float ay = __uint_as_float((meta << 15) & 0x7f800000u) * inv.y;
+  const float az = __uint_as_float((meta << 7) & 0x7f800000u) * inv.z;
   const float bx = fmaf(__uint_as_float(q0.x), inv.x, -oi.x);
   const float by = fmaf(__uint_as_float(q0.y), inv.y, -oi.y);
   const float bz = fmaf(__uint_as_float(q0.z), inv.z, -oi.z);
@@ -266,12 +291,12 @@ __device__ __forceinline__ void node4_boxes(const tparams& P, const trav_state& 
   }
 }
 
+// Slot mask of the hit leaf children.
 __device__ __forceinline__ uint32_t leaf_mask(const node4_hits& H) {
   uint32_t leaves = 0;
-  leaves |= (H.hit[0] && H.ch.x < 0) ? 1u : 0u;
-  leaves |= (H.hit[1] && H.ch.y < 0) ? 2u : 0u;
-  leaves |= (H.hit[2] && H.ch.z < 0) ? 4u : 0u;
-  leaves |= (H.hit[3] && H.ch.w < 0) ? 8u : 0u;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    leaves |= (H.hit[c] && (uint32_t)c >= H.nint && (uint32_t)c < H.nchild) ? (1u << c) : 0u;
   return leaves;
 }
 
@@ -284,11 +309,10 @@ __device__ __forceinline__ uint32_t leaf_mask(const node4_hits& H) {
 // tested in between), so entry distances are compared with it again.
 template <int LS, bool RECHECK>
 __device__ __forceinline__ bool order_push(const tparams& P, trav_state& T, const node4_hits& H, uint64_t* ovf) {
-  const int4 ch = H.ch;
-  uint32_t k0 = sort_key(H.hit[0] && ch.x >= 0 && (!RECHECK || H.tmin[0] <= T.tbest), H.tmin[0], 0);
-  uint32_t k1 = sort_key(H.hit[1] && ch.y >= 0 && (!RECHECK || H.tmin[1] <= T.tbest), H.tmin[1], 1);
-  uint32_t k2 = sort_key(H.hit[2] && ch.z >= 0 && (!RECHECK || H.tmin[2] <= T.tbest), H.tmin[2], 2);
-  uint32_t k3 = sort_key(H.hit[3] && ch.w >= 0 && (!RECHECK || H.tmin[3] <= T.tbest), H.tmin[3], 3);
+  uint32_t k0 = sort_key(H.hit[0] && 0u < H.nint && (!RECHECK || H.tmin[0] <= T.tbest), H.tmin[0], 0);
+  uint32_t k1 = sort_key(H.hit[1] && 1u < H.nint && (!RECHECK || H.tmin[1] <= T.tbest), H.tmin[1], 1);
+  uint32_t k2 = sort_key(H.hit[2] && 2u < H.nint && (!RECHECK || H.tmin[2] <= T.tbest), H.tmin[2], 2);
+  uint32_t k3 = sort_key(H.hit[3] && 3u < H.nint && (!RECHECK || H.tmin[3] <= T.tbest), H.tmin[3], 3);
   {
     uint32_t a = min(k0, k1), b = max(k0, k1);
     k0 = a; k1 = b;
@@ -307,12 +331,12 @@ __device__ __forceinline__ bool order_push(const tparams& P, trav_state& T, cons
   for (int i = 0; i < 3; ++i) {
     if (ks[i] == 0xffffffffu) continue;
     // entry: node in the low word, entry distance bits in the high word
-    const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | sel4(ch, ks[i] & 3u);
+    const uint64_t e = ((uint64_t)(ks[i] & ~3u) << 32) | child_ref(H, H.contiguous, ks[i] & 3u);
     if (T.sp < LS) *lds_slot((uint32_t)T.sp, P.block) = e;
     else ovf[T.sp - LS] = e;
     ++T.sp;
   }
-  T.cur = (int)sel4(ch, k0 & 3u);
+  T.cur = (int)child_ref(H, H.contiguous, k0 & 3u);
   T.curd = __uint_as_float(k0 & ~3u);
   return true;
 }
@@ -345,16 +369,18 @@ __device__ __forceinline__ bool pop4(const tparams& P, trav_state& T, uint64_t* 
 // One four-wide node per call, leaf children tested right away (before the
 // internal children are ordered, so they are culled with the updated closest
 // hit). Returns true when the query is finished.
-template <int LS, bool COUNT>
+template <int LS, bool COUNT, bool MESH>
 __device__ __forceinline__ bool trav4_step(const tparams& P, const f3 o, const f3 d, trav_state& T, uint64_t* ovf,
                                            counters& C) {
   node4_hits H;
-  node4_boxes<COUNT>(P, T, H, C);
+  node4_boxes<COUNT, MESH>(P, T, H, C);
   uint32_t leaves = leaf_mask(H);
-  while (leaves) {
-    const uint32_t c = (uint32_t)(__ffs(leaves) - 1);
-    leaves &= leaves - 1;
-    test_prim<COUNT>(P, o, d, ~sel4(H.ch, c), T, C);
+  if (leaves) {
+    do {
+      const uint32_t c = (uint32_t)(__ffs(leaves) - 1);
+      leaves &= leaves - 1;
+      test_prim<COUNT>(P, o, d, leaf_prim(H, c), T, C);
+    } while (leaves);
   }
   if (order_push<LS, true>(P, T, H, ovf)) return false;
   if (pop4<LS>(P, T, ovf)) return false;
@@ -443,20 +469,29 @@ __device__ __forceinline__ uint64_t hit_key(float t, uint32_t p) {
 // Node phase: visits T.cur unless its entry distance lies beyond the closest
 // hit found since it was chosen, descends into the nearest hit internal
 // child or pops (T.cur = -1 when no node is left). Returns the visit's leaf
-// group: slot c holds `owner | ~child` for a hit leaf child (owner: bit 31
+// group: slot c holds `owner | primitive` for a hit leaf child (owner: bit 31
 // and the lane in bits 25..30), else 0.
-template <int LS, bool COUNT>
+template <int LS, bool COUNT, bool MESH>
 __device__ __forceinline__ uint4 node4_visit(const tparams& P, trav_state& T, uint64_t* ovf, uint32_t owner,
                                              counters& C) {
   uint4 g = make_uint4(0u, 0u, 0u, 0u);
   if (T.curd <= T.tbest) {
     node4_hits H;
-    node4_boxes<COUNT>(P, T, H, C);
-    const int4 ch = H.ch;
-    g.x = (H.hit[0] && ch.x < 0) ? (owner | ~(uint32_t)ch.x) : 0u;
-    g.y = (H.hit[1] && ch.y < 0) ? (owner | ~(uint32_t)ch.y) : 0u;
-    g.z = (H.hit[2] && ch.z < 0) ? (owner | ~(uint32_t)ch.z) : 0u;
-    g.w = (H.hit[3] && ch.w < 0) ? (owner | ~(uint32_t)ch.w) : 0u;
+    node4_boxes<COUNT, MESH>(P, T, H, C);
+    const uint32_t lm = leaf_mask(H);
+    if (H.contiguous) {
+      // leaf c is leaf_info + c - nint (no carry into the owner bits: ids < 2^25)
+      const uint32_t lb = (owner | H.leaf_info) - H.nint;
+      g.x = (lm & 1u) ? lb : 0u;
+      g.y = (lm & 2u) ? lb + 1u : 0u;
+      g.z = (lm & 4u) ? lb + 2u : 0u;
+      g.w = (lm & 8u) ? lb + 3u : 0u;
+    } else if (lm) {
+      g.x = (lm & 1u) ? (owner | H.lp.x) : 0u;
+      g.y = (lm & 2u) ? (owner | H.lp.y) : 0u;
+      g.z = (lm & 4u) ? (owner | H.lp.z) : 0u;
+      g.w = (lm & 8u) ? (owner | H.lp.w) : 0u;
+    }
     if (order_push<LS, false>(P, T, H, ovf)) return g;
   }
   if (!pop4<LS>(P, T, ovf)) T.cur = -1;
@@ -529,11 +564,11 @@ __device__ __forceinline__ uint32_t leaf_phase(const tparams& P, const f3 ro, co
 
 // Per-lane traversal stack of either tree: LS entries in LDS, the rest in a
 // private array.
-template <int LS, int W>
+template <int LS, int W, bool MESH>
 struct lane_stack;
 
-template <int LS>
-struct lane_stack<LS, 2> {
+template <int LS, bool MESH>
+struct lane_stack<LS, 2, MESH> {
   uint32_t ovf[64 - LS];
   template <bool COUNT>
   __device__ __forceinline__ bool step(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
@@ -541,16 +576,16 @@ struct lane_stack<LS, 2> {
   }
 };
 
-template <int LS>
-struct lane_stack<LS, 4> {
+template <int LS, bool MESH>
+struct lane_stack<LS, 4, MESH> {
   uint64_t ovf[kWideStack - LS];
   template <bool COUNT>
   __device__ __forceinline__ bool step(const tparams& P, const f3 o, const f3 d, trav_state& T, counters& C) {
-    return trav4_step<LS, COUNT>(P, o, d, T, ovf, C);
+    return trav4_step<LS, COUNT, MESH>(P, o, d, T, ovf, C);
   }
   template <bool COUNT>
   __device__ __forceinline__ uint4 visit(const tparams& P, trav_state& T, uint32_t owner, counters& C) {
-    return node4_visit<LS, COUNT>(P, T, ovf, owner, C);
+    return node4_visit<LS, COUNT, MESH>(P, T, ovf, owner, C);
   }
 };
 
@@ -737,7 +772,7 @@ __device__ __forceinline__ void start_ray(const tparams& P, uint64_t gid, ray_st
 // query, which starts at the instance's root.
 // qseq: the wave's leaf-list head sequence number (postponed leaves; no
 // entry of this lane is pending).
-template <int W>
+template <int W, bool MESH>
 __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, trav_state& T, counters& C,
                                               uint32_t qseq) {
   const inst_desc& D = P.desc[R.inst];
@@ -757,7 +792,8 @@ __device__ __forceinline__ void begin_segment(const tparams& P, ray_state& R, tr
   T.tbest = T.tb;
   T.found = false;
   T.pbest = 0;
-  T.cur = W == 4 ? D.root4 : D.root;
+  // the root of a tree of discs loads its fourth word (it is nobody's child)
+  T.cur = W == 4 ? (int)((uint32_t)D.root4 | (!MESH && D.prim_kind == 0 ? kNodeHasLeaves : 0u)) : D.root;
   T.curd = 0.0f;
   T.sp = 0;
   T.lastq = qseq;
@@ -902,10 +938,10 @@ __device__ __forceinline__ void flush_counters(const tparams& P, const counters&
 // entries it would take reach leaf_min / 64 of the lanes with a node left,
 // when the list could not take another node phase's leaves, or when no node
 // is left. A lane's query is open while it has a node or a listed leaf left.
-template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT, bool PL>
+template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT, bool PL, bool MESH>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ? 6 : 7))) void k_trace_persistent(
     tparams P) {
-  lane_stack<LS, W> S;
+  lane_stack<LS, W, MESH> S;
   counters C;
   ray_state R;
   trav_state T;
@@ -953,7 +989,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
       if (!alive && !exhausted) {
         if (my < P.ray_hi) {
           start_ray(P, my, R);
-          begin_segment<W>(P, R, T, C, lseq);
+          begin_segment<W, MESH>(P, R, T, C, lseq);
           alive = true;
           tracing = true;
         } else {
@@ -1024,7 +1060,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
     if (alive && !tracing) {
       alive = finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
       if (alive) {
-        begin_segment<W>(P, R, T, C, lseq);
+        begin_segment<W, MESH>(P, R, T, C, lseq);
         tracing = true;
       }
     }
@@ -1033,9 +1069,9 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(ADJ ?
 }
 
 // Plain trace: one thread per ray, every segment traversed to the end.
-template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT>
+template <int LS, int W, bool ADJ, bool FP64ACC, bool COUNT, bool MESH>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
-  lane_stack<LS, W> S;
+  lane_stack<LS, W, MESH> S;
   counters C;
   ray_state R;
   trav_state T;
@@ -1043,7 +1079,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
   if (gid < P.ray_hi) {
     start_ray(P, gid, R);
     for (;;) {
-      begin_segment<W>(P, R, T, C, 0u);
+      begin_segment<W, MESH>(P, R, T, C, 0u);
       while (!S.template step<COUNT>(P, R.o, R.d, T, C)) {
       }
       if (!finish_segment<ADJ, FP64ACC, COUNT>(P, R, T, C)) break;
@@ -1054,42 +1090,50 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_plain(tparams P) {
 
 using kfn = void (*)(tparams);
 
-template <int LS, int W, bool PL>
+template <int LS, int W, bool PL, bool MESH>
 kfn pick_persistent(bool adj, bool fp64, bool count) {
-  if (count) return k_trace_persistent<LS, W, false, false, true, PL>;
-  if (adj) return fp64 ? k_trace_persistent<LS, W, true, true, false, PL> : k_trace_persistent<LS, W, true, false, false, PL>;
-  return fp64 ? k_trace_persistent<LS, W, false, true, false, PL> : k_trace_persistent<LS, W, false, false, false, PL>;
+  if (count) return k_trace_persistent<LS, W, false, false, true, PL, MESH>;
+  if (adj)
+    return fp64 ? k_trace_persistent<LS, W, true, true, false, PL, MESH>
+                : k_trace_persistent<LS, W, true, false, false, PL, MESH>;
+  return fp64 ? k_trace_persistent<LS, W, false, true, false, PL, MESH>
+              : k_trace_persistent<LS, W, false, false, false, PL, MESH>;
 }
 
-template <int LS, int W>
+template <int LS, int W, bool MESH>
 kfn pick(bool persistent, bool postpone, bool adj, bool fp64, bool count) {
   if (persistent) {
     if constexpr (W == 4) {
-      if (postpone) return pick_persistent<LS, W, true>(adj, fp64, count);
+      if (postpone) return pick_persistent<LS, W, true, MESH>(adj, fp64, count);
     }
-    return pick_persistent<LS, W, false>(adj, fp64, count);
+    return pick_persistent<LS, W, false, MESH>(adj, fp64, count);
   }
-  if (count) return k_trace_plain<LS, W, false, false, true>;
-  if (adj) return fp64 ? k_trace_plain<LS, W, true, true, false> : k_trace_plain<LS, W, true, false, false>;
-  return fp64 ? k_trace_plain<LS, W, false, true, false> : k_trace_plain<LS, W, false, false, false>;
+  if (count) return k_trace_plain<LS, W, false, false, true, MESH>;
+  if (adj) return fp64 ? k_trace_plain<LS, W, true, true, false, MESH> : k_trace_plain<LS, W, true, false, false, MESH>;
+  return fp64 ? k_trace_plain<LS, W, false, true, false, MESH> : k_trace_plain<LS, W, false, false, false, MESH>;
 }
 
-template <int W>
+template <int W, bool MESH>
 kfn pick_ls(int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count) {
   switch (lds_stack) {
-    case 4: return pick<4, W>(persistent, postpone, adj, fp64, count);
-    case 6: return pick<6, W>(persistent, postpone, adj, fp64, count);
-    case 8: return pick<8, W>(persistent, postpone, adj, fp64, count);
-    case 16: return pick<16, W>(persistent, postpone, adj, fp64, count);
-    case 24: return pick<24, W>(persistent, postpone, adj, fp64, count);
-    case 32: return pick<32, W>(persistent, postpone, adj, fp64, count);
+    case 4: return pick<4, W, MESH>(persistent, postpone, adj, fp64, count);
+    case 6: return pick<6, W, MESH>(persistent, postpone, adj, fp64, count);
+    case 8: return pick<8, W, MESH>(persistent, postpone, adj, fp64, count);
+    case 16: return pick<16, W, MESH>(persistent, postpone, adj, fp64, count);
+    case 24: return pick<24, W, MESH>(persistent, postpone, adj, fp64, count);
+    case 32: return pick<32, W, MESH>(persistent, postpone, adj, fp64, count);
     default: return nullptr;
   }
 }
 
-kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count) {
-  if (width == 2) return pick_ls<2>(lds_stack, persistent, false, adj, fp64, count);
-  if (width == 4) return pick_ls<4>(lds_stack, persistent, postpone, adj, fp64, count);
+// mesh: every instance of the launch is a triangle mesh (leaves contiguous
+// in every node; the four-wide kernels then drop the disc clouds' leaf word).
+kfn pick_kernel(int width, int lds_stack, bool persistent, bool postpone, bool adj, bool fp64, bool count,
+                bool mesh) {
+  if (width == 2) return pick_ls<2, false>(lds_stack, persistent, false, adj, fp64, count);
+  if (width == 4)
+    return mesh ? pick_ls<4, true>(lds_stack, persistent, postpone, adj, fp64, count)
+                : pick_ls<4, false>(lds_stack, persistent, postpone, adj, fp64, count);
   return nullptr;
 }
 
@@ -1140,7 +1184,7 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
   const int bs = L.block_size;
   if (bs < 64 || bs > kMaxBlock || (bs & 63)) return hipErrorInvalidValue;
   P.block = (uint32_t)bs;
-  const kfn f = pick_kernel(L.width, L.lds_stack, persistent, postpone, L.adjoint, L.accum_fp64, count);
+  const kfn f = pick_kernel(L.width, L.lds_stack, persistent, postpone, L.adjoint, L.accum_fp64, count, L.mesh != 0);
   if (!f) return hipErrorInvalidValue;
   const size_t shmem = trace_lds_bytes(L.width, bs, L.lds_stack, postpone);
   uint64_t grid;
@@ -1155,9 +1199,9 @@ hipError_t launch_trace(const trace_launch& L, hipStream_t stream) {
 }
 
 hipError_t trace_occupancy(int lds_stack, int block_size, int persistent, int leaf_batch, int adjoint, int fp64,
-                           int width, int* blocks_per_cu) {
+                           int width, int mesh, int* blocks_per_cu) {
   const bool postpone = persistent && width == 4 && leaf_batch > 0;
-  const kfn f = pick_kernel(width, lds_stack, persistent, postpone, adjoint, fp64, false);
+  const kfn f = pick_kernel(width, lds_stack, persistent, postpone, adjoint, fp64, false, mesh != 0);
   if (!f) return hipErrorInvalidValue;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block_size,
                                                       trace_lds_bytes(width, block_size, lds_stack, postpone));

@@ -1,7 +1,10 @@
 """The four-wide traversal tree (quantised child boxes) and its traversal.
 
 * structure: every node reachable once, every primitive exactly one leaf,
-  1..4 children per node;
+  1..4 children per node, internal children first, siblings contiguous; a
+  mesh's primitives stored in the tree's depth-first order (each node's
+  leaves contiguous from leaf_info), a disc cloud's leaf_info marking the
+  internal children with leaves;
 * conservativeness: each child's box, decoded exactly as the kernel decodes it
   (org + q * 2^(e - 127) rounded once to float32), encloses the exact bounds
   of every primitive below it -- compared without tolerance;
@@ -13,7 +16,7 @@ import numpy as np
 import pytest
 
 from rti_amd import Context, geometry, run_scene
-from rti_amd._lib import wide_scales
+from rti_amd._lib import wide_children, wide_scales
 from tests.helpers import small_hole_mesh, small_trench
 
 pytestmark = pytest.mark.gpu
@@ -51,43 +54,51 @@ def _validate_wide(wide, perm, scene):
     seen_prim = np.zeros(n, dtype=np.int64)
     seen_node = np.zeros(len(wide), dtype=np.int64)
     order, stack = [], [0]
+    leaf_order = []  # leaves in depth-first order, each node's leaves before its subtrees
+    mesh = scene["kind"] != "discs"
     while stack:
         v = stack.pop()
         seen_node[v] += 1
         order.append(v)
         _, m = wide_scales(wide[v])
         for c in range(m, 4):
-            # empty slot: inverted box, and a leaf (primitive) as the child
+            # empty slot: inverted box
             assert all((int(wide["lo"][v][k]) >> (8 * c)) & 0xFF == 0xFF for k in range(3))
             assert all((int(wide["hi"][v][k]) >> (8 * c)) & 0xFF == 0 for k in range(3))
-            assert int(wide["child"][v][c]) < 0
-        for c in range(m):
-            ch = int(wide["child"][v][c])
-            if ch >= 0:
-                stack.append(ch)
-            else:
-                seen_prim[~ch] += 1
+        kids = wide_children(wide[v])
+        internal = [ch for ch in kids if ch >= 0]
+        leaves = [~ch for ch in kids if ch < 0]
+        # internal children first, then the leaves
+        assert kids == internal + [~p for p in leaves]
+        for p in leaves:
+            seen_prim[p] += 1
+        leaf_order.extend(leaves)
+        if mesh and leaves:
+            assert leaves == list(range(int(wide["leaf_info"][v]), int(wide["leaf_info"][v]) + len(leaves)))
+        if not mesh:
+            marks = sum(1 << c for c, ch in enumerate(internal) if any(k < 0 for k in wide_children(wide[ch])))
+            assert int(wide["leaf_info"][v]) == marks, f"node {v}: leaf marks"
+        stack.extend(reversed(internal))
     reached = seen_node > 0
     assert np.all(seen_node[reached] == 1), "every four-wide node is reached exactly once"
     # the slots no node reaches are padding: an inverted box in all four slots
     for v in np.nonzero(~reached)[0]:
         assert all(int(wide["lo"][v][k]) == 0xFFFFFFFF and int(wide["hi"][v][k]) == 0 for k in range(3)), v
-    # every family of siblings starts on an even slot (a 128-byte line)
+    # every family of siblings starts on an even slot (a 128-byte line) and is contiguous
     for v in order:
-        _, m = wide_scales(wide[v])
-        kids = [int(wide["child"][v][c]) for c in range(m) if int(wide["child"][v][c]) >= 0]
+        kids = [ch for ch in wide_children(wide[v]) if ch >= 0]
         if kids:
             assert min(kids) % 2 == 0, f"node {v}: family at odd slot {min(kids)}"
             assert kids == list(range(min(kids), min(kids) + len(kids))), "siblings are adjacent"
     assert (~reached).sum() <= reached.sum() + 1
     assert np.all(seen_prim == 1), f"{np.sum(seen_prim != 1)} primitives are not exactly one leaf"
+    if mesh:
+        assert leaf_order == list(range(n)), "a mesh's primitives are stored in the tree's depth-first order"
     sub_lo = np.full((len(wide), 3), np.inf)
     sub_hi = np.full((len(wide), 3), -np.inf)
     bad = 0
     for v in reversed(order):
-        _, m = wide_scales(wide[v])
-        for c in range(m):
-            ch = int(wide["child"][v][c])
+        for c, ch in enumerate(wide_children(wide[v])):
             blo, bhi = (plo[~ch], phi[~ch]) if ch < 0 else (sub_lo[ch], sub_hi[ch])
             qlo, qhi = _decode(wide[v], c)
             if np.any(qlo > blo) or np.any(qhi < bhi):
@@ -96,10 +107,6 @@ def _validate_wide(wide, perm, scene):
                     print("node", v, "child", c, "->", ch, "box", qlo, qhi, "needs", blo, bhi)
             sub_lo[v] = np.minimum(sub_lo[v], blo)
             sub_hi[v] = np.maximum(sub_hi[v], bhi)
-        for c in range(m, 4):
-            p = ~int(wide["child"][v][c])
-            assert np.all(plo[p] >= sub_lo[v]) and np.all(phi[p] <= sub_hi[v]), \
-                f"node {v}: the empty slot's leaf {p} lies outside the subtree"
     assert bad == 0, f"{bad} quantised child boxes do not enclose their subtree"
 
 
