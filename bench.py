@@ -119,13 +119,16 @@ def bytes_per_segment(counters):
     return b, parts
 
 
-def record_bytes_per_segment(parts, prim_bytes, deposit_bytes):
-    """The same visits priced at the sizes of this layout's records: 64 B per
-    four-wide node, prim_bytes per primitive test (48 B triangle, 32 B disc),
-    36 B per neighbour test, deposit_bytes per deposit (read + write of the
-    8-byte sums: three, five with the adjoint), no ray-state traffic (rays
-    stay in registers)."""
-    return 64.0 * parts["V_node"] + prim_bytes * parts["V_prim"] + 36.0 * parts["K_nb"] + deposit_bytes * parts["D"]
+def record_bytes_per_segment(parts, prim_bytes, deposit_bytes, node_bytes):
+    """The same visits priced at the sizes of this layout's records:
+    node_bytes per four-wide node visit (the 48 bytes a visit loads; a disc
+    cloud's marked nodes load their 16-byte leaf word too, priced here at 64),
+    prim_bytes per primitive test (48 B triangle, 32 B disc), 36 B per
+    neighbour test, deposit_bytes per deposit (read + write of the 8-byte
+    sums: three, five with the adjoint), no ray-state traffic (rays stay in
+    registers)."""
+    return (node_bytes * parts["V_node"] + prim_bytes * parts["V_prim"] + 36.0 * parts["K_nb"]
+            + deposit_bytes * parts["D"])
 
 
 def cpu_model():
@@ -281,7 +284,10 @@ def profile_figure(name, key):
     (rocprof's VALUBusy: SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE of
     one XCC; above 1 when two VALU instructions issue in one quad-cycle),
     "valu_dual_issue" (SQ_ACTIVE_INST_VALU2 / SQ_ACTIVE_INST_VALU),
-    "active_lanes" (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64)."""
+    "active_lanes" (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU, of 64),
+    "td_busy" / "ta_busy" (TD_TD_BUSY / TA_TA_BUSY summed over the CUs,
+    over CU_NUM x GRBM_GUI_ACTIVE of one XCC: the texture data and address
+    units of the vector-memory path)."""
     path = os.path.join(ROOT, "profiles", f"trace_persistent_{name}_pmc.md")
     if not os.path.exists(path):
         return None
@@ -380,7 +386,7 @@ def main():
     cnt_ctx.batch_run()
     B, parts = bytes_per_segment(cnt_ctx.counters())
     cnt_ctx.close()
-    B_rec = record_bytes_per_segment(parts, prim_bytes, 80.0 if adj else 48.0)
+    B_rec = record_bytes_per_segment(parts, prim_bytes, 80.0 if adj else 48.0, 48.0 if prim_bytes == 48.0 else 64.0)
     achieved = value * B / 1e9
     b_meas = profile_figure(args.workload, "bytes_per_segment")
     traffic = None if b_meas is None else b_meas * segments
@@ -388,14 +394,20 @@ def main():
     valu_busy = profile_figure(args.workload, "valu_busy")
     dual = profile_figure(args.workload, "valu_dual_issue")
     lanes = profile_figure(args.workload, "active_lanes")
+    td_busy = profile_figure(args.workload, "td_busy")
+    ta_busy = profile_figure(args.workload, "ta_busy")
     # the resource that binds, from the committed counters of this workload:
-    # HBM when the traffic reaches 0.7 of the peak; VALU issue when the SIMDs
-    # are nearly always issuing VALU work (VALUBusy >= 0.85) below that;
+    # HBM when the traffic reaches 0.7 of the peak; below that the vector-
+    # memory path (texture data unit busy >= 0.85 of the time: divergent
+    # loads, about one cycle per lane per load instruction); else VALU issue
+    # when the SIMDs are nearly always issuing VALU work (VALUBusy >= 0.85);
     # otherwise neither is saturated: latency and the launch's tail
     binding = None
     if valu_busy is not None and traffic_gbs is not None:
         if traffic_gbs >= 0.7 * HBM_PEAK_GBS:
             binding = "hbm"
+        elif td_busy is not None and td_busy >= 0.85:
+            binding = "vmem"
         elif valu_busy >= 0.85:
             binding = "valu"
         else:
@@ -407,15 +419,17 @@ def main():
                 "l2_hit_rate": profile_figure(args.workload, "l2_hit_rate"),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
                 "bytes_per_segment_records": round(B_rec, 1),
-                "binding": binding, "valu_busy": valu_busy, "valu_dual_issue": dual, "active_lanes": lanes,
+                "binding": binding, "td_busy": td_busy, "ta_busy": ta_busy, "valu_busy": valu_busy,
+                "valu_dual_issue": dual, "active_lanes": lanes,
                 "lane_frac": None if lanes is None else round(lanes / 64, 3),
                 "note": "achieved and frac: B_seg = 32 V_node + 32 V_prim + 24 D + 36 K_nb + 64 bytes per segment "
                         "(V_node four-wide node visits) x segments/s over the HBM peak; bytes_per_segment_records "
                         "prices the same visits at this layout's record sizes. traffic, traffic_gbs, l2_hit_rate, "
                         "valu_busy (rocprof VALUBusy; above 1 with dual VALU issue), valu_dual_issue and "
-                        "active_lanes (of 64, per issued VALU instruction) come from the committed counter summary "
-                        "of the trace kernel on this workload (null without one); binding names the resource "
-                        "those counters show binding (hbm, valu, or latency when neither is saturated)"}
+                        "active_lanes (of 64, per issued VALU instruction), td_busy and ta_busy (texture data and "
+                        "address units) come from the committed counter summary of the trace kernel on this "
+                        "workload (null without one); binding names the resource those counters show binding "
+                        "(hbm, vmem: the vector-memory path's data unit, valu, or latency when none is saturated)"}
 
     cpu = None
     if not args.no_cpu:
