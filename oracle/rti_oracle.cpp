@@ -814,7 +814,9 @@ void trace_range(const scene_view<R>& s, const orc_params& p, thread_acc& acc, u
         w = (R)0.3;
       }
       o = add(o, mul(d, tp));  // meta_geometry::get_new_origin (src/rti/geo/meta_geometry.hpp:17-23)
-      if (u01(u.v[3]) < p.diffuse_prob) d = diffuse_dir<R>(n, u01(u.v[1]), u01(u.v[2]));
+      // cos^2 from (0, 1]: a draw of 0 would leave the ray in the tangent
+      // plane of the surface it left (cos_hemi.hpp:25-26 draws 64 bits)
+      if (u01(u.v[3]) < p.diffuse_prob) d = diffuse_dir<R>(n, u01(u.v[1]), u01_open0(u.v[2]));
       else d = specular_dir<R>(d, n);
     }
   }

@@ -880,7 +880,13 @@ __device__ __forceinline__ bool shade_segment(const tparams& P, ray_state& R, co
     R.w = 0.3f;
   }
   R.o = R.o + R.d * tp;  // meta_geometry::get_new_origin (src/rti/geo/meta_geometry.hpp:17-23)
-  if (u01(u.w) < D.diffuse_prob) R.d = diffuse_dir(n, u01(u.y), u01(u.z));
+  // cos^2 of the diffuse direction is drawn from (0, 1]: from [0, 1) a
+  // 24-bit draw of 0 (one bounce in 2^24) would give a direction exactly in
+  // the surface's tangent plane, a ray that travels inside the plane of the
+  // surface it left and, on a flat periodic top surface, never meets it again
+  // (tens of thousands of boundary segments); the reference's 64-bit draws
+  // (cos_hemi.hpp:25-26 over mt19937_64) make that a 2^-64 event
+  if (u01(u.w) < D.diffuse_prob) R.d = diffuse_dir(n, u01(u.y), u01_open0(u.z));
   else R.d = R.d - n * (2.0f * dot(R.d, n));  // specular::use (src/rti/reflection/specular.hpp:16-40)
   return true;
 }
