@@ -396,17 +396,21 @@ def main():
     lanes = profile_figure(args.workload, "active_lanes")
     td_busy = profile_figure(args.workload, "td_busy")
     ta_busy = profile_figure(args.workload, "ta_busy")
+    cu_busy = profile_figure(args.workload, "cu_busy")
     # the resource that binds, from the committed counters of this workload:
     # HBM when the traffic reaches 0.7 of the peak; below that the vector-
-    # memory path (texture data unit busy >= 0.85 of the time: divergent
-    # loads, about one cycle per lane per load instruction); else VALU issue
-    # when the SIMDs are nearly always issuing VALU work (VALUBusy >= 0.85);
-    # otherwise neither is saturated: latency and the launch's tail
+    # memory path (texture data unit busy >= 0.85 of the time the CUs are
+    # busy: divergent loads, about one cycle per lane per load instruction);
+    # else VALU issue when the SIMDs are nearly always issuing VALU work
+    # (VALUBusy >= 0.85); otherwise neither is saturated: latency and the
+    # launch's tail. cu_busy (SQ_BUSY_CU_CYCLES / CU_NUM / GRBM_GUI_ACTIVE of
+    # one XCC) is the part of the launch before its drain; 1 where the
+    # summary does not give it.
     binding = None
     if valu_busy is not None and traffic_gbs is not None:
         if traffic_gbs >= 0.7 * HBM_PEAK_GBS:
             binding = "hbm"
-        elif td_busy is not None and td_busy >= 0.85:
+        elif td_busy is not None and td_busy >= 0.85 * (cu_busy if cu_busy is not None else 1.0):
             binding = "vmem"
         elif valu_busy >= 0.85:
             binding = "valu"
@@ -419,7 +423,8 @@ def main():
                 "l2_hit_rate": profile_figure(args.workload, "l2_hit_rate"),
                 "bytes_per_segment": round(B, 1), **{k: round(v, 3) for k, v in parts.items()},
                 "bytes_per_segment_records": round(B_rec, 1),
-                "binding": binding, "td_busy": td_busy, "ta_busy": ta_busy, "valu_busy": valu_busy,
+                "binding": binding, "td_busy": td_busy, "ta_busy": ta_busy, "cu_busy": cu_busy,
+                "valu_busy": valu_busy,
                 "valu_dual_issue": dual, "active_lanes": lanes,
                 "lane_frac": None if lanes is None else round(lanes / 64, 3),
                 "note": "achieved and frac: B_seg = 32 V_node + 32 V_prim + 24 D + 36 K_nb + 64 bytes per segment "
@@ -427,9 +432,10 @@ def main():
                         "prices the same visits at this layout's record sizes. traffic, traffic_gbs, l2_hit_rate, "
                         "valu_busy (rocprof VALUBusy; above 1 with dual VALU issue), valu_dual_issue and "
                         "active_lanes (of 64, per issued VALU instruction), td_busy and ta_busy (texture data and "
-                        "address units) come from the committed counter summary of the trace kernel on this "
-                        "workload (null without one); binding names the resource those counters show binding "
-                        "(hbm, vmem: the vector-memory path's data unit, valu, or latency when none is saturated)"}
+                        "address units), cu_busy (CUs busy over the launch; the rest is its drain) come from the "
+                        "committed counter summary of the trace kernel on this workload (null without one); "
+                        "binding names the resource those counters show binding while the CUs are busy (hbm, "
+                        "vmem: the vector-memory path's data unit, valu, or latency when none is saturated)"}
 
     cpu = None
     if not args.no_cpu:

@@ -310,6 +310,60 @@ def test_tuner_descent_ignores_gains_within_noise(monkeypatch):
                                   + 2 * len(tune.LEAF) + len(tune.REFILL))
 
 
+def test_tuner_full_size_decides_between_close_candidates(monkeypatch):
+    """When the candidates were timed on a sample, the ones the sample ranks
+    within FINAL_BAND of the best are timed again on the full workload, and
+    the full-size rate decides: a leaf schedule 0.3 % ahead on the sample but
+    6 % behind at full size (the launch's drain) is not kept. The kernel is
+    replaced by a rate model, so no GPU is used."""
+    from rti_amd import tune
+
+    class FakeContext:
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def batch_set(self, items):
+            self.rays = sum(it["n_rays"] for it in items)
+
+        def prepare(self):
+            pass
+
+    calls = []
+
+    def rate(ctx, knobs, timed=2):
+        full = ctx.rays == 4_000_000
+        calls.append((full, dict(knobs)))
+        r = 5.0e9
+        for name, v in (("compact_lanes", 56), ("waves_per_cu", 0), ("lds_stack", 8), ("block_size", 256),
+                        ("refill_lanes", 8)):
+            if knobs[name] != v:
+                r *= 0.97  # clear preferences, outside the band: never re-timed
+        if knobs["leaf_batch"] == 128:
+            r *= 0.94 if full else 1.003
+        elif knobs["leaf_batch"] != -1:
+            r *= 0.97
+        return r
+
+    monkeypatch.setattr(tune, "Context", FakeContext)
+    monkeypatch.setattr(tune, "_rate", rate)
+    scene = {"kind": "triangles", "tris": np.zeros((4, 3), np.uint32), "verts": np.zeros((3, 3), np.float32)}
+    out = tune.tune([{"scene": scene, "n_rays": 4_000_000}], fraction=0.25)
+    k = tune.knobs(out)
+    assert k["leaf_batch"] != 128
+    assert k["leaf_batch"] == -1 and out["segments_per_s"] == 5.0e9
+    full = [c for f, c in calls if f]
+    assert len(full) == 2
+    assert any(c["leaf_batch"] == 128 for c in full) and any(c["leaf_batch"] == -1 for c in full)
+    assert all((c["compact_lanes"], c["lds_stack"], c["refill_lanes"]) == (56, 8, 8) for c in full)
+    # a workload timed in full is not re-timed
+    calls.clear()
+    out = tune.tune([{"scene": scene, "n_rays": 4_000_000}], fraction=1.0)
+    assert out["full_trials"] == []
+
+
 def test_tuner_cache_key_covers_geometry_physics_and_device():
     """The tuner's cache key changes with the geometry's contents (same
     size), the boundary conditions, the source and the device."""
