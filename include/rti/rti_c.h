@@ -156,6 +156,8 @@ typedef struct rti_stats {
   uint64_t wide_nodes;    /* four-wide traversal-tree nodes                                  */
   int32_t fixed_shift2;   /* fixed-point scale of the sum of squared deposits is 2^fixed_shift2 */
   int32_t pad0;
+  uint64_t max_segments;  /* segments of the longest ray path of the last trace (the paths
+                             that set how long a launch's last waves run)               */
 } rti_stats;
 
 /* One geometry of a batch. Exactly one of (xyzr, nrm) or (verts, tris) is set.

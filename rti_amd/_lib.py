@@ -71,6 +71,7 @@ class Stats(ctypes.Structure):
         ("trace_ms", ctypes.c_double), ("build_ms", ctypes.c_double), ("post_ms", ctypes.c_double),
         ("box", ctypes.c_float * 6), ("src_z", ctypes.c_float), ("wide_stack", ctypes.c_uint32),
         ("wide_nodes", ctypes.c_uint64), ("fixed_shift2", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("max_segments", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -111,7 +111,7 @@ struct rti_ctx {
   uint64_t acc_total = 0;  // accumulator slots: the primitives of every instance
   double build_ms = 0, trace_ms = 0, post_ms = 0;
   int last_adjoint = 0, last_fp64 = 0, last_count = 0, last_moments = 0;
-  uint64_t run_stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t run_stats[rti_impl::kStatsWords] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::string graph_key;
@@ -124,7 +124,7 @@ namespace {
 // The no-allocation trace keeps its instance descriptors, ray queue and
 // counters in this file-scope device array: one copy per GPU, held until the
 // process ends, shared by every such call on that GPU (calls that overlap on
-// one GPU race on it). Layout: [0, 8) ray queue, [64, 128) the eight
+// one GPU race on it). Layout: [0, 8) ray queue, [64, 136) the nine
 // counters, [256, ...) up to kNoAllocMaxInstances descriptors. In host code
 // the name `scratch` is a host-side stand-in, not a device address; host code
 // uses only the per-device address resolved by resolve_scratch.
@@ -133,6 +133,7 @@ constexpr size_t kScratchStatsOffset = 64;
 constexpr size_t kScratchDescOffset = 256;
 constexpr size_t kScratchBytes = kScratchDescOffset + kNoAllocMaxInstances * sizeof(inst_desc);  // 33,024 B
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static_assert(kScratchStatsOffset + 8 * rti_impl::kStatsWords <= kScratchDescOffset, "counters overlap the descriptors");
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
@@ -412,7 +413,7 @@ int prepare(rti_ctx* ctx) {
   HIP_TRY(ctx, ctx->areas.ensure(np * 4));
   HIP_TRY(ctx, ctx->acc.ensure(na * 8 * rti_dev::kAccStride));
   HIP_TRY(ctx, ctx->queue.ensure(64));
-  HIP_TRY(ctx, ctx->stats.ensure(64));
+  HIP_TRY(ctx, ctx->stats.ensure(rti_impl::kStatsWords * 8));
   HIP_TRY(ctx, ctx->desc.ensure(ctx->inst.size() * sizeof(inst_desc)));
   rti_impl::store_ptrs st{ctx->prims.as<float4>(), ctx->prim_shift, ctx->perm.as<uint32_t>(), ctx->nodes.as<bvh_node>(), ctx->nodes4.as<bvh4_node>(),
                           ctx->nb_ptr.as<uint32_t>(), nullptr, ctx->areas.as<float>()};
@@ -670,7 +671,7 @@ struct trace_bufs {
   unsigned long long* acc;
   inst_desc* desc;
   unsigned long long* queue;
-  unsigned long long* stats;  // 8 counters
+  unsigned long long* stats;  // kStatsWords counters
 };
 
 trace_bufs ctx_bufs(rti_ctx* ctx) {
@@ -688,7 +689,7 @@ int enqueue_trace(rti_ctx* ctx, const trace_bufs& B, const kernel_cfg& cfg, bool
   const bool mom = ctx->params.moments != 0;
   if (mom)  // 0.0 has all bits zero
     HIP_TRY(ctx, rti_impl::fill_u64(ctx->mom.as<unsigned long long>(), np * 2, 0ull, ctx->stream));
-  HIP_TRY(ctx, rti_impl::fill_u64(B.stats, 8, 0ull, ctx->stream));
+  HIP_TRY(ctx, rti_impl::fill_u64(B.stats, rti_impl::kStatsWords, 0ull, ctx->stream));
   rti_impl::trace_launch L{};
   L.desc = B.desc;
   L.n_inst = (uint32_t)ctx->inst.size();
@@ -760,7 +761,7 @@ int trace(rti_ctx* ctx) {
   ctx->last_fp64 = fp64;
   ctx->last_count = count;
   ctx->last_moments = ctx->params.moments != 0;
-  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, sizeof ctx->run_stats, hipMemcpyDeviceToHost));
   ctx->have_results = false;
   ctx->acc_traced = true;
   return RTI_OK;
@@ -904,6 +905,7 @@ void fill_stats(rti_ctx* ctx, const instance* I, rti_stats* out) {
   out->surface_hits = ctx->run_stats[1];
   out->misses = ctx->run_stats[2];
   out->truncated = ctx->run_stats[3];
+  out->max_segments = ctx->run_stats[8];
   out->n_edges = I ? I->n_edges : ne;
   out->bvh_depth = I ? I->depth : depth;
   const instance& J = I ? *I : ctx->inst[0];
@@ -1324,7 +1326,7 @@ int rti_batch_replay(rti_ctx* ctx, uint32_t times) {
   ctx->last_fp64 = fp64;
   ctx->last_count = 0;
   ctx->last_moments = ctx->params.moments != 0;
-  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, 64, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, ctx->stats.p, sizeof ctx->run_stats, hipMemcpyDeviceToHost));
   ctx->have_results = false;
   if (times) ctx->acc_traced = true;
   return RTI_OK;
@@ -1371,7 +1373,7 @@ int rti_trace_noalloc(rti_ctx* ctx, unsigned long long* acc, uint64_t acc_words)
   ctx->last_fp64 = fp64;
   ctx->last_count = count;
   ctx->last_moments = 0;
-  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, B.stats, 64, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(ctx->run_stats, B.stats, sizeof ctx->run_stats, hipMemcpyDeviceToHost));
   // the context's getters and rti_finalize refer to its own accumulators,
   // which this trace did not write
   ctx->have_results = false;

@@ -87,6 +87,11 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uin
 size_t scan_tmp_size(uint32_t n);
 
 // ---- trace ----
+// Run counters of a trace: segments, surface hits, misses, truncated rays,
+// (instrumented runs) node visits, primitive tests, neighbour tests,
+// deposits, then the segments of the longest ray path.
+constexpr int kStatsWords = 9;
+
 struct trace_launch {
   const rti_dev::inst_desc* desc;   // n_inst descriptors
   uint32_t n_inst;
@@ -101,7 +106,7 @@ struct trace_launch {
   unsigned long long* adj;          // 2 per slot: d/ds, d/dn (signed fixed point); may be null
   double* mom;                      // 2 per slot: sums of deposit^3, ^4 (FP64 atomics); may be null
   unsigned long long* queue;        // [0]: next ray id
-  unsigned long long* stats;        // segments, surface hits, misses, truncated
+  unsigned long long* stats;        // kStatsWords run counters
   double fx_scale;                  // 2^shift
   int adjoint;
   int accum_fp64;

@@ -66,6 +66,7 @@ struct tparams {
 struct counters {
   uint32_t seg = 0, hit = 0, miss = 0, trunc = 0;
   uint32_t nodes = 0, prims = 0, nbt = 0, deps = 0;
+  uint32_t longest = 0;  // segments of the longest path this lane ended
 };
 
 // Resumable closest-hit traversal (one node per trav_step), so that a wave
@@ -896,6 +897,7 @@ __device__ __forceinline__ bool finish_segment(const tparams& P, ray_state& R, c
   const bool more = shade_segment<ADJ, FP64ACC, COUNT>(P, R, T, C);
   const bool cap = more && R.segs >= kMaxSegmentsPerRay;
   C.trunc += cap ? 1u : 0u;
+  if (!more || cap) C.longest = R.segs > C.longest ? R.segs : C.longest;
   return more && !cap;
 }
 
@@ -910,6 +912,15 @@ __device__ __forceinline__ void flush_one(const tparams& P, int i, uint32_t v) {
 
 template <bool COUNT>
 __device__ __forceinline__ void flush_counters(const tparams& P, const counters& C) {
+  {
+    uint32_t m = C.longest;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)m, off, 64);
+      m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(P.stats + 8, (unsigned long long)m);
+  }
   flush_one(P, 0, C.seg);
   flush_one(P, 1, C.hit);
   flush_one(P, 2, C.miss);

@@ -262,6 +262,30 @@ def test_flat_plate_flux_is_one():
     assert g["stats"]["misses"] < 1e-3 * n
 
 
+def test_diffuse_draw_never_leaves_along_the_surface():
+    """Regression: cos^2 of the diffuse lobe drawn from a 24-bit uniform in
+    [0, 1) was exactly 0 once in 2^24 bounces, a direction in the surface's
+    tangent plane; on a flat periodic plate such a ray travelled inside the
+    plane of the discs it left, never meeting one, and wrapped through the
+    boundaries for tens of thousands of segments (one such path set the
+    launch time of a 64-instance batch). Ray 19,050,278 of seed 1, instance 0
+    draws that word at its first reflection (Philox counter (ray, 0, 1, 0),
+    word 2 = 219): with the draw from [0, 1) this run's longest path had
+    103,123 segments. Drawn from (0, 1], its cos^2 is 2^-24 and it climbs out
+    of the box (about 0.03 high) within about a hundred boundary crossings:
+    the longest path of the run (rti_stats.max_segments) has 84."""
+    w = oracle.philox([19_050_278, 0, 1, 0], [1, 0])
+    assert int(w[2]) >> 8 == 0
+    sc = geometry.flat_plate_discs(size=2.0, spacing=0.02)
+    n = 19_050_278 + 50_000
+    g = _gpu(sc, n, 0.1, seed=1, instance=0, diffuse_prob=1.0, bc_x=1, bc_y=1)
+    st = g["stats"]
+    print(f"[gpu] longest path {st['max_segments']} segments, {st['segments']} in all", flush=True)
+    assert st["truncated"] == 0
+    assert 2 <= st["max_segments"] < 300
+    assert st["segments"] < 2.2 * n
+
+
 def test_mirror_trench_method_of_images():
     """All-mirror triangle trench without rejection control (diffuse_prob = 0,
     roulette off): every path is a straight line through the mirror images,
