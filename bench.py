@@ -456,7 +456,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc + "; one persistent launch per step, HIP graph",
-                   "rays_per_step": rays_per_step, "segments_per_step": segments, "knobs": knobs},
+                   "rays_per_step": rays_per_step, "segments_per_step": segments,
+                   "longest_path_segments": int(st["max_segments"]), "knobs": knobs},
         "plain": plain,
         "batched_vs_plain": (value / plain["segments_per_s"]) if plain else None,
         "roofline": roofline,
