@@ -943,9 +943,12 @@ __device__ __forceinline__ void flush_counters(const tparams& P, const counters&
 //     reservation of queue ids, topped up by one atomic per wave when it runs
 //     short (ballot + popcount assign consecutive ids).
 // Register budget: 7 resident waves per SIMD (72 VGPRs) measured 16 % faster
-// than the unconstrained 88-VGPR allocation on the triangle-hole workload and
-// spills nothing; the adjoint variant carries four more live values per ray
-// and gets 6 (80 VGPRs). The launch can still run fewer waves per CU
+// than the unconstrained 88-VGPR allocation on the triangle-hole workload;
+// the compiler spills about 20 registers, all outside the traversal loop
+// (around shading and the disc-neighbour loop, once per shading pass), and
+// that loop's only scratch traffic is the stack beyond its LDS entries. The
+// adjoint variant carries four more live values per ray and gets 6 (80
+// VGPRs). The launch can still run fewer waves per CU
 // (waves_per_cu) where the memory system prefers it.
 //
 // With postponed leaves (PL, four-wide tree) each traversal iteration is
