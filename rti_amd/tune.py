@@ -10,7 +10,9 @@ then the block size, then ``leaf_batch`` (-1: leaf tests at each node visit;
 32..256: postponed leaf tests on the wave's shared leaf list, a leaf phase
 once the entries it takes reach that many 64ths of the lanes with a node
 left), then ``refill_lanes``
-(lanes of a wave that must wait for a new ray before the wave refills). When
+(lanes of a wave that must wait for a new ray before the wave refills); for
+workloads of disc clouds only, the binary-tree topology (``tree_builder``,
+LBVH or PLOC) first. When
 the candidates were timed on a sample, the few that the sample ranks within
 1 % of the best are timed again on the full workload, which decides. Results
 are cached in a JSON file keyed by the batch signature, so a repeated
@@ -30,6 +32,11 @@ BLOCK = (128, 256)
 LEAF = (-1, 64, 128, 256)
 REFILL = (1, 8, 16)
 WAVES = (0, 20, 24)
+# binary-tree topology (RTI_TREE_LBVH, RTI_TREE_PLOC): tuned for workloads of
+# disc clouds only, whose results do not depend on it (a triangle mesh's
+# stored order follows its tree, so the tree decides exact ties at shared
+# edges, and a mesh keeps the default)
+TREE = (1, 2)
 MIN_GAIN = 0.005  # relative gain a coordinate step must show to change a knob
 FINAL_BAND = 0.01  # sample-rate band of the candidates re-timed on the full workload
 FINALISTS = 4
@@ -66,17 +73,25 @@ def _rate(ctx, knobs, timed=2):
 
 
 def _finalists(passes, best):
-    """The descent's result and, fastest on the sample first, single-knob
-    variants of it: each knob set to a value that its last coordinate pass
-    ranked within FINAL_BAND of that pass's best; at most FINALISTS in all."""
+    """The descent's result, the other tree topologies (their effect on the
+    launch's longest paths does not show on a sample: on the 64-instance disc
+    batch PLOC was 2 % slower on the sample and 3-5 % faster at full size),
+    and, fastest on the sample first, single-knob variants: each knob set to
+    a value that its last coordinate pass ranked within FINAL_BAND of that
+    pass's best; at most FINALISTS in all."""
+    out = [dict(best)]
+    for v in passes.get("tree_builder", {}):
+        if v != best["tree_builder"]:
+            out.append(dict(best, tree_builder=v))
     cands = []
     for name, scores in passes.items():
+        if name == "tree_builder":
+            continue
         top = max(scores.values())
         for v, r in scores.items():
             if v != best[name] and r >= top * (1.0 - FINAL_BAND):
                 cands.append((r, name, v))
     cands.sort(key=lambda c: -c[0])
-    out = [dict(best)]
     for r, name, v in cands:
         if len(out) >= FINALISTS:
             break
@@ -96,7 +111,8 @@ def default_fraction(items, sample_rays=2e8, full_up_to=2e8):
 
 
 def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
-    """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size, leaf_batch, refill_lanes} and their rate."""
+    """Best knobs {compact_lanes, waves_per_cu, lds_stack, block_size, leaf_batch, refill_lanes, and
+    tree_builder for disc clouds} and their rate."""
     key = signature(items) if cache else None
     if cache and os.path.exists(cache):
         with open(cache) as fh:
@@ -108,6 +124,12 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     small = [dict(it, n_rays=max(min_rays, int(it["n_rays"] * fraction))) for it in items]
     best = {"compact_lanes": 56, "waves_per_cu": 0, "lds_stack": 8, "block_size": 256, "leaf_batch": 0,
             "refill_lanes": 1}
+    discs = all(it["scene"]["kind"] == "discs" for it in items)
+    order = [("compact_lanes", COMPACT), ("waves_per_cu", WAVES), ("lds_stack", LDS), ("block_size", BLOCK),
+             ("leaf_batch", LEAF), ("refill_lanes", REFILL), ("block_size", BLOCK), ("leaf_batch", LEAF)]
+    if discs:
+        best["tree_builder"] = 1  # the default for disc clouds, moved only for a clear gain
+        order.insert(0, ("tree_builder", TREE))
     trials = []
     passes = {}  # knob -> {value: sample rate} of its last coordinate pass
     with Context() as ctx:
@@ -115,9 +137,7 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
         ctx.prepare()
         # one pass over all knobs, then the block size and the leaf-phase
         # threshold once more (the two interact most)
-        for name, values in (("compact_lanes", COMPACT), ("waves_per_cu", WAVES), ("lds_stack", LDS),
-                             ("block_size", BLOCK), ("leaf_batch", LEAF), ("refill_lanes", REFILL),
-                             ("block_size", BLOCK), ("leaf_batch", LEAF)):
+        for name, values in order:
             scores = {}
             for v in values:
                 knobs = dict(best, **{name: v})
@@ -173,4 +193,4 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
 
 def knobs(tuned):
     return {k: tuned[k] for k in ("compact_lanes", "waves_per_cu", "lds_stack", "block_size", "leaf_batch",
-                                  "refill_lanes") if k in tuned}
+                                  "refill_lanes", "tree_builder") if k in tuned}

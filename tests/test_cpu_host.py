@@ -364,6 +364,51 @@ def test_tuner_full_size_decides_between_close_candidates(monkeypatch):
     assert out["full_trials"] == []
 
 
+def test_tuner_picks_tree_topology_for_disc_clouds_only(monkeypatch):
+    """The binary-tree topology is a tuned knob only when every item is a disc
+    cloud (their results do not depend on it); a triangle mesh, whose stored
+    order and exact edge ties follow its tree, keeps the default. The kernel is
+    replaced by a rate model, so no GPU is used."""
+    from rti_amd import tune
+
+    class FakeContext:
+        def __enter__(self):
+            return self
+
+        def __exit__(self, *a):
+            return False
+
+        def batch_set(self, items):
+            pass
+
+        def prepare(self):
+            pass
+
+    seen = []
+
+    def rate(ctx, knobs, timed=2):
+        seen.append(dict(knobs))
+        return 5.0e9 * (1.05 if knobs.get("tree_builder") == 2 else 1.0)
+
+    monkeypatch.setattr(tune, "Context", FakeContext)
+    monkeypatch.setattr(tune, "_rate", rate)
+    discs = {"kind": "discs", "xyzr": np.zeros((4, 4), np.float32), "normals": np.zeros((4, 3), np.float32)}
+    mesh = {"kind": "triangles", "tris": np.zeros((4, 3), np.uint32), "verts": np.zeros((3, 3), np.float32)}
+    out = tune.tune([{"scene": discs, "n_rays": 1_000_000}], fraction=1.0)
+    assert tune.knobs(out)["tree_builder"] == 2
+    assert {k.get("tree_builder") for k in seen} == {1, 2}
+    # timed on a sample, the other topology is always timed again at full size
+    seen.clear()
+    out = tune.tune([{"scene": discs, "n_rays": 4_000_000}], fraction=0.25)
+    full = [k for k in seen[-8:] if k.get("tree_builder") == 1]
+    assert tune.knobs(out)["tree_builder"] == 2 and full and len(out["full_trials"]) >= 2
+    assert {t["tree_builder"] for t in out["full_trials"]} == {1, 2}
+    seen.clear()
+    out = tune.tune([{"scene": discs, "n_rays": 1_000_000}, {"scene": mesh, "n_rays": 1_000_000}], fraction=1.0)
+    assert "tree_builder" not in tune.knobs(out)
+    assert all("tree_builder" not in k for k in seen)
+
+
 def test_tuner_cache_key_covers_geometry_physics_and_device():
     """The tuner's cache key changes with the geometry's contents (same
     size), the boundary conditions, the source and the device."""
