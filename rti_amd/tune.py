@@ -14,7 +14,8 @@ left), then ``refill_lanes``
 workloads of disc clouds only, the binary-tree topology (``tree_builder``,
 LBVH or PLOC) first. When
 the candidates were timed on a sample, the few that the sample ranks within
-1 % of the best are timed again on the full workload, which decides. Results
+1 % of the best, and for disc clouds the other tree topology, are timed again
+on the full workload, which decides. Results
 are cached in a JSON file keyed by the batch signature, so a repeated
 workload is not re-tuned.
 """
@@ -159,10 +160,10 @@ def tune(items, fraction=None, min_rays=20000, cache=None, log=None):
     full_trials = []
     if fraction < 1.0:
         # The sample does not reproduce the full launch's drain (paths of
-        # thousands of segments end it), so the descent's result and its
-        # single-knob variants that the sample ranks within FINAL_BAND of it
-        # are timed once more on the full workload, and the fastest there is
-        # kept: on the 64-instance disc batch, leaf tests at each node visit
+        # thousands of segments end it), so the descent's result, the other
+        # tree topology (disc clouds) and its single-knob variants that the
+        # sample ranks within FINAL_BAND of it are timed once more on the
+        # full workload, and the fastest there is kept: on the 64-instance disc batch, leaf tests at each node visit
         # and leaf_batch 128 were 0.2 % apart on a 2e8-ray sample and 6 %
         # apart at its full 6.4e8 rays.
         finalists = _finalists(passes, best)
